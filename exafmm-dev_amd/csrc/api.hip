@@ -1,0 +1,364 @@
+// C-ABI of libexafmm_hip (declared in include/exafmm_hip.h).  Every entry
+// point converts exceptions into status codes and keeps the message on the
+// handle.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+using exafmm::Error;
+using exafmm::Handle;
+
+struct exafmm_hip_handle {
+  Handle h;
+};
+
+namespace exafmm {
+
+// SetRopt (include/up_down_pass.h:15-42,125-129).  powf/std::pow of the host
+// C library are used so that the radii -- and hence the interaction lists --
+// round as in the reference; the device keeps everything else.
+__global__ void k_mass_normalise(cplx *__restrict__ M, int ncells, int nt) {
+  const int c = blockIdx.x;
+  const double m0 = M[(size_t)c * nt].re;  // M[0] is real for real charges
+  for (int t = 1 + threadIdx.x; t < nt; t += blockDim.x) {
+    cplx v = M[(size_t)c * nt + t];
+    v.re /= m0;
+    v.im /= m0;
+    M[(size_t)c * nt + t] = v;
+  }
+}
+
+void run_ropt(Handle &h) {
+  if (!h.cfg.useRopt) return;
+  const int nc = h.ncells, nt = h.nterm, P = h.cfg.P;
+  const double theta = h.cfg.theta;
+  std::vector<double> m0(nc);
+  std::vector<double4> X(nc);
+  EXA_HIP(hipMemcpy2DAsync(m0.data(), sizeof(double), h.M.p, nt * sizeof(cplx), sizeof(double), nc,
+                           hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(X.data(), h.cX.p, nc * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  const double c0 = (1 - theta) * (1 - theta) / std::pow(theta, P + 2) / powf((float)std::fabs(m0[0]), (float)(1.0 / 3));
+  if (h.cfg.mass) hipLaunchKernelGGL(k_mass_normalise, dim3(nc), dim3(64), 0, h.stream, h.M.p, nc, nt);
+  for (int c = 0; c < nc; c++) {
+    double x = 1.0 / theta;
+    const double a = c0 * powf((float)std::fabs(m0[c]), (float)(1.0 / 3));
+    for (int i = 0; i < 5; i++) {
+      const double f = x * x - 2 * x + 1 - a * std::pow(x, -P);
+      const double df = (P + 2) * x - 2 * (P + 1) + P / x;
+      x -= f / df;
+    }
+    X[c].w *= x * theta;
+  }
+  EXA_HIP(hipMemcpyAsync(h.cX.p, X.data(), nc * sizeof(double4), hipMemcpyHostToDevice, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+}
+
+__global__ void k_scatter_results(const double4 *__restrict__ trg, const int *__restrict__ bidx, int n,
+                                  double4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[bidx[i]] = trg[i];
+}
+
+static void ensure(bool cond, const char *msg) {
+  if (!cond) throw Error(EXAFMM_HIP_EINVAL, msg);
+}
+
+static void set_device(Handle &h) { EXA_HIP(hipSetDevice(h.cfg.device)); }
+
+static void do_build(Handle &h) {
+  ensure(h.have_bounds, "build_tree: bounds not computed");
+  ensure(h.n > 0, "build_tree: no bodies");
+  if (h.cfg.builder == 0)
+    run_build_uniform(h);
+  else
+    run_build_adaptive(h);
+  h.have_tree = true;
+  h.have_up = h.have_trav = false;
+}
+
+static void do_upward(Handle &h) {
+  ensure(h.have_tree, "upward_pass: tree not built");
+  run_upward(h);
+  run_ropt(h);
+  h.have_up = true;
+}
+
+static void do_traverse(Handle &h) {
+  ensure(h.have_up, "traverse: upward pass not run");
+  EXA_HIP(hipEventRecord(h.ev[4], h.stream));
+  run_lists(h);
+  EXA_HIP(hipEventRecord(h.ev[5], h.stream));
+  run_p2p(h);
+  EXA_HIP(hipEventRecord(h.ev[6], h.stream));
+  run_m2l(h);
+  if (h.cfg.images > 0) run_periodic(h);
+  EXA_HIP(hipEventRecord(h.ev[7], h.stream));
+  h.have_trav = true;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+}  // namespace exafmm
+
+using namespace exafmm;
+
+template <typename F>
+static int guarded(exafmm_hip_handle *H, F &&f) {
+  if (!H) return EXAFMM_HIP_EINVAL;
+  try {
+    set_device(H->h);
+    f(H->h);
+    H->h.err.clear();
+    return EXAFMM_HIP_OK;
+  } catch (const Error &e) {
+    H->h.err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    H->h.err = e.what();
+    return EXAFMM_HIP_EDEVICE;
+  }
+}
+
+extern "C" {
+
+int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
+  if (!cfg || !out) return EXAFMM_HIP_EINVAL;
+  *out = nullptr;
+  if (cfg->P != 4 && cfg->P != 6 && cfg->P != 8 && cfg->P != 10 && cfg->P != 12) return EXAFMM_HIP_EUNSUP;
+  if (cfg->theta <= 0 || cfg->theta >= 1 || cfg->ncrit <= 0 || cfg->images < 0 || cfg->images > 8)
+    return EXAFMM_HIP_EINVAL;
+  if (cfg->builder != 0 && cfg->builder != 1) return EXAFMM_HIP_EINVAL;
+  exafmm_hip_handle *H = new exafmm_hip_handle();
+  H->h.cfg = *cfg;
+  if (H->h.cfg.nspawn <= 0) H->h.cfg.nspawn = 1000;
+  H->h.nterm = cfg->P * (cfg->P + 1) / 2;
+  try {
+    set_device(H->h);
+    EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
+    for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));
+  } catch (const Error &e) {
+    delete H;
+    return e.code;
+  }
+  *out = H;
+  return EXAFMM_HIP_OK;
+}
+
+int exafmm_hip_destroy(exafmm_hip_handle *H) {
+  if (!H) return EXAFMM_HIP_EINVAL;
+  (void)hipSetDevice(H->h.cfg.device);
+  if (H->h.stream) (void)hipStreamSynchronize(H->h.stream);
+  delete H;
+  return EXAFMM_HIP_OK;
+}
+
+const char *exafmm_hip_last_error(const exafmm_hip_handle *H) { return H ? H->h.err.c_str() : "null handle"; }
+
+int exafmm_hip_set_bodies(exafmm_hip_handle *H, int n, const double *xyzq) {
+  return guarded(H, [&](Handle &h) {
+    ensure(n >= 0 && (n == 0 || xyzq), "set_bodies: bad arguments");
+    h.n = n;
+    h.in4.resize(n > 0 ? n : 1);
+    if (n) EXA_HIP(hipMemcpyAsync(h.in4.p, xyzq, n * sizeof(double4), hipMemcpyHostToDevice, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    h.have_bounds = h.have_tree = h.have_up = h.have_trav = false;
+  });
+}
+
+int exafmm_hip_set_bodies_device(exafmm_hip_handle *H, int n, const double *d_xyzq) {
+  return guarded(H, [&](Handle &h) {
+    ensure(n >= 0 && (n == 0 || d_xyzq), "set_bodies_device: bad arguments");
+    h.n = n;
+    h.in4.resize(n > 0 ? n : 1);
+    if (n) EXA_HIP(hipMemcpyAsync(h.in4.p, d_xyzq, n * sizeof(double4), hipMemcpyDeviceToDevice, h.stream));
+    h.have_bounds = h.have_tree = h.have_up = h.have_trav = false;
+  });
+}
+
+int exafmm_hip_get_bounds(exafmm_hip_handle *H, double *xmin3, double *xmax3) {
+  return guarded(H, [&](Handle &h) {
+    run_bounds(h);
+    for (int d = 0; d < 3; d++) {
+      if (xmin3) xmin3[d] = h.xmin[d];
+      if (xmax3) xmax3[d] = h.xmax[d];
+    }
+  });
+}
+
+int exafmm_hip_build_tree(exafmm_hip_handle *H, const double *xmin3, const double *xmax3, int *ncells) {
+  return guarded(H, [&](Handle &h) {
+    if (xmin3 && xmax3) {
+      for (int d = 0; d < 3; d++) {
+        h.xmin[d] = xmin3[d];
+        h.xmax[d] = xmax3[d];
+      }
+      h.have_bounds = true;
+    }
+    do_build(h);
+    if (ncells) *ncells = h.ncells;
+  });
+}
+
+int exafmm_hip_upward_pass(exafmm_hip_handle *H) { return guarded(H, [&](Handle &h) { do_upward(h); }); }
+
+int exafmm_hip_traverse(exafmm_hip_handle *H) { return guarded(H, [&](Handle &h) { do_traverse(h); }); }
+
+int exafmm_hip_downward_pass(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_trav, "downward_pass: traversal not run");
+    run_downward(h);
+  });
+}
+
+int exafmm_hip_evaluate(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) {
+    EXA_HIP(hipEventRecord(h.ev[0], h.stream));
+    run_bounds(h);
+    EXA_HIP(hipEventRecord(h.ev[1], h.stream));
+    do_build(h);
+    EXA_HIP(hipEventRecord(h.ev[2], h.stream));
+    do_upward(h);
+    EXA_HIP(hipEventRecord(h.ev[3], h.stream));
+    do_traverse(h);
+    run_downward(h);
+    EXA_HIP(hipEventRecord(h.ev[8], h.stream));
+    EXA_HIP(hipEventSynchronize(h.ev[8]));
+    exafmm_hip_stats &s = h.stats;
+    s.t_bounds = elapsed(h.ev[0], h.ev[1]);
+    s.t_build = elapsed(h.ev[1], h.ev[2]);
+    s.t_upward = elapsed(h.ev[2], h.ev[3]);
+    s.t_lists = elapsed(h.ev[4], h.ev[5]);
+    s.t_p2p = elapsed(h.ev[5], h.ev[6]);
+    s.t_m2l = elapsed(h.ev[6], h.ev[7]);
+    s.t_downward = elapsed(h.ev[7], h.ev[8]);
+    s.t_total = elapsed(h.ev[0], h.ev[8]);
+  });
+}
+
+int exafmm_hip_get_cells(exafmm_hip_handle *H, int *ints5, uint64_t *icell, double *X, double *R, double *M,
+                         double *L) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_tree, "get_cells: no tree");
+    const int nc = h.ncells;
+    std::vector<int> a(nc);
+    DBuf<int> *cols[5] = {&h.cParent, &h.cChild, &h.cNChild, &h.cBody, &h.cNBody};
+    if (ints5) {
+      for (int k = 0; k < 5; k++) {
+        EXA_HIP(hipMemcpyAsync(a.data(), cols[k]->p, nc * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+        EXA_HIP(hipStreamSynchronize(h.stream));
+        for (int c = 0; c < nc; c++) ints5[5 * c + k] = a[c];
+      }
+    }
+    if (icell) EXA_HIP(hipMemcpyAsync(icell, h.cKey.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, h.stream));
+    if (X || R) {
+      std::vector<double4> x(nc);
+      EXA_HIP(hipMemcpyAsync(x.data(), h.cX.p, nc * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+      EXA_HIP(hipStreamSynchronize(h.stream));
+      for (int c = 0; c < nc; c++) {
+        if (X) {
+          X[3 * c] = x[c].x;
+          X[3 * c + 1] = x[c].y;
+          X[3 * c + 2] = x[c].z;
+        }
+        if (R) R[c] = x[c].w;
+      }
+    }
+    const size_t nb = (size_t)nc * h.nterm * sizeof(cplx);
+    if (M && h.M.n >= (size_t)nc * h.nterm) EXA_HIP(hipMemcpyAsync(M, h.M.p, nb, hipMemcpyDeviceToHost, h.stream));
+    if (L && h.L.n >= (size_t)nc * h.nterm) EXA_HIP(hipMemcpyAsync(L, h.L.p, nb, hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+int exafmm_hip_get_bodies(exafmm_hip_handle *H, double *xyzq, int *ibody, double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_tree, "get_bodies: no tree");
+    if (xyzq) EXA_HIP(hipMemcpyAsync(xyzq, h.body4.p, h.n * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    if (ibody) EXA_HIP(hipMemcpyAsync(ibody, h.bidx.p, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    if (trg) EXA_HIP(hipMemcpyAsync(trg, h.trg4.p, h.n * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+int exafmm_hip_get_results(exafmm_hip_handle *H, double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_trav && trg, "get_results: nothing evaluated");
+    DBuf<double4> o;
+    o.resize(h.n);
+    hipLaunchKernelGGL(k_scatter_results, dim3(div_up(h.n, 256)), dim3(256), 0, h.stream, h.trg4.p, h.bidx.p, h.n, o.p);
+    EXA_HIP(hipGetLastError());
+    EXA_HIP(hipMemcpyAsync(trg, o.p, h.n * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    o.release();
+  });
+}
+
+int exafmm_hip_results_device(exafmm_hip_handle *H, const double **d_trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(d_trg != nullptr, "results_device: null out pointer");
+    *d_trg = (const double *)h.trg4.p;
+  });
+}
+
+int exafmm_hip_get_list_counts(exafmm_hip_handle *H, int64_t *np2p, int64_t *nm2l) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_trav, "get_list_counts: traversal not run");
+    if (np2p) *np2p = h.np2p;
+    if (nm2l) *nm2l = h.nm2l;
+  });
+}
+
+int exafmm_hip_get_lists(exafmm_hip_handle *H, int *p2p_i, int *p2p_j, int *m2l_i, int *m2l_j) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_trav, "get_lists: traversal not run");
+    const int nc = h.ncells;
+    auto expand = [&](DBuf<int> &off, DBuf<int> &src, int64_t np, int *ti, int *tj) {
+      if (np == 0) return;
+      std::vector<int> o(nc + 1);
+      EXA_HIP(hipMemcpyAsync(o.data(), off.p, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+      if (tj) EXA_HIP(hipMemcpyAsync(tj, src.p, np * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+      EXA_HIP(hipStreamSynchronize(h.stream));
+      if (ti)
+        for (int c = 0; c < nc; c++)
+          for (int e = o[c]; e < o[c + 1]; e++) ti[e] = c;
+    };
+    expand(h.p2p_off, h.p2p_src, h.np2p, p2p_i, p2p_j);
+    expand(h.m2l_off, h.m2l_src, h.nm2l, m2l_i, m2l_j);
+  });
+}
+
+int exafmm_hip_direct(exafmm_hip_handle *H, int nt, const int *target_ibody, double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(nt >= 0 && (nt == 0 || (target_ibody && trg)), "direct: bad arguments");
+    run_direct(h, nt, target_ibody, trg);
+  });
+}
+
+int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
+  return guarded(H, [&](Handle &h) {
+    ensure(s != nullptr, "get_stats: null");
+    h.stats.num_p2p_pairs = h.np2p;
+    h.stats.num_m2l_pairs = h.nm2l;
+    h.stats.num_cells = h.ncells;
+    h.stats.num_levels = h.nlevels;
+    if (h.have_trav) h.stats.num_p2p_bodies = count_p2p_body_pairs(h);
+    *s = h.stats;
+  });
+}
+
+int exafmm_hip_synchronize(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) { EXA_HIP(hipStreamSynchronize(h.stream)); });
+}
+
+}  // extern "C"
+
+namespace exafmm {
+void run_build_adaptive(Handle &) { throw Error(EXAFMM_HIP_EUNSUP, "adaptive builder not available yet"); }
+void run_periodic(Handle &) { throw Error(EXAFMM_HIP_EUNSUP, "periodic images not available yet"); }
+}  // namespace exafmm

@@ -1,0 +1,116 @@
+// Device-side solid harmonics for the Laplace spherical basis.
+//
+// Conventions (those of kernels/LaplaceSphericalCPU.cxx:29-101):
+//   regular  R_n^m = (-1)^n rho^n / (n+m)!  P_n^m(cos a) e^{i m b}
+//   singular S_n^m = (-1)^n (n-m)! rho^{-n-1} P_n^m(cos a) e^{i m b}
+// with P_n^m the associated Legendre function including the Condon-Shortley
+// phase (P_m^m = (-1)^m (2m-1)!! sin^m a) and X_n^{-m} = conj(X_n^m).
+// Coefficient index of (n, m >= 0) in a packed expansion: n(n+1)/2 + m.
+//
+// The GPU takes cos a = z/r and e^{ib} = (x + i y)/sqrt(x^2 + y^2) straight
+// from the Cartesian offset instead of going through acos/atan2.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace exafmm {
+
+template <int P>
+struct Expn {
+  static constexpr int NT = P * (P + 1) / 2;  // packed terms, m >= 0
+};
+
+__host__ __device__ constexpr int tri(int n) { return n * (n + 1) / 2; }
+
+struct Geom {
+  double r;     // |dX|
+  double ct;    // cos(alpha)
+  double st;    // sin(alpha) >= 0
+  double cp;    // cos(beta)
+  double sp;    // sin(beta)
+};
+
+__device__ inline Geom make_geom(double dx, double dy, double dz) {
+  Geom g;
+  const double rxy2 = dx * dx + dy * dy;
+  g.r = sqrt(rxy2 + dz * dz);
+  const double rxy = sqrt(rxy2);
+  if (g.r == 0) {
+    g.ct = 1;
+    g.st = 0;
+  } else {
+    g.ct = dz / g.r;
+    g.st = rxy / g.r;
+  }
+  if (rxy == 0) {
+    g.cp = 1;
+    g.sp = 0;
+  } else {
+    g.cp = dx / rxy;
+    g.sp = dy / rxy;
+  }
+  return g;
+}
+
+// 1/k! for k < 32.
+__device__ inline double inv_factorial(int k) {
+  double f = 1;
+  for (int i = 2; i <= k; i++) f /= i;
+  return f;
+}
+
+__device__ inline double factorial(int k) {
+  double f = 1;
+  for (int i = 2; i <= k; i++) f *= i;
+  return f;
+}
+
+// Associated Legendre P_n^m(ct) for one (n, m >= 0), upward recurrence in n.
+__device__ inline double legendre(int n, int m, double ct, double st) {
+  double pmm = 1;
+  for (int i = 1; i <= m; i++) pmm *= -(2 * i - 1) * st;
+  double p = pmm, p1 = 0;
+  for (int k = m; k < n; k++) {
+    const double pn = (ct * (2 * k + 1) * p - (k + m) * p1) / (k - m + 1);
+    p1 = p;
+    p = pn;
+  }
+  return p;
+}
+
+// e^{i m b} by repeated multiplication.
+__device__ inline void eim(int m, double cp, double sp, double &c, double &s) {
+  c = 1;
+  s = 0;
+  for (int i = 0; i < m; i++) {
+    const double t = c * cp - s * sp;
+    s = c * sp + s * cp;
+    c = t;
+  }
+}
+
+// One regular harmonic R_n^m, m >= 0.
+__device__ inline void regular_nm(int n, int m, const Geom &g, double &re, double &im) {
+  double sc = inv_factorial(n + m);
+  for (int i = 0; i < n; i++) sc *= g.r;
+  if (n & 1) sc = -sc;
+  const double p = legendre(n, m, g.ct, g.st) * sc;
+  double c, s;
+  eim(m, g.cp, g.sp, c, s);
+  re = p * c;
+  im = p * s;
+}
+
+// One singular harmonic S_n^m, m >= 0.
+__device__ inline void singular_nm(int n, int m, const Geom &g, double &re, double &im) {
+  const double inv_r = 1.0 / g.r;
+  double sc = factorial(n - m) * inv_r;
+  for (int i = 0; i < n; i++) sc *= inv_r;
+  if (n & 1) sc = -sc;
+  const double p = legendre(n, m, g.ct, g.st) * sc;
+  double c, s;
+  eim(m, g.cp, g.sp, c, s);
+  re = p * c;
+  im = p * s;
+}
+
+}  // namespace exafmm

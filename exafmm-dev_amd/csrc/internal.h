@@ -1,0 +1,142 @@
+// Shared internals of libexafmm_hip: the handle, device buffers, error
+// plumbing and launch helpers.  Device code lives in the *.hip files; this
+// header is included by all of them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/exafmm_hip.h"
+
+namespace exafmm {
+
+// Error carried from deep inside the library to the C-ABI boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define EXA_HIP(call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::exafmm::Error(EXAFMM_HIP_EDEVICE, std::string(#call " failed: ") +            \
+                                                    hipGetErrorString(e_) + " at " __FILE__); \
+  } while (0)
+
+// Grow-only device buffer.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  size_t n = 0;
+  void reserve(size_t want) {
+    if (want <= cap) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t c = want + want / 4 + 64;
+    if (hipMalloc(&p, c * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      throw Error(EXAFMM_HIP_ENOMEM, "hipMalloc of " + std::to_string(c * sizeof(T)) + " bytes failed");
+    }
+    cap = c;
+  }
+  void resize(size_t want) {
+    reserve(want);
+    n = want;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Double-precision complex stored as two doubles (re, im).
+struct cplx {
+  double re, im;
+};
+
+struct Handle {
+  exafmm_hip_config cfg;
+  int nterm = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[16] = {};
+
+  // Bodies.
+  int n = 0;
+  DBuf<double4> in4;     // caller order (x, y, z, q)
+  DBuf<double4> body4;   // tree order
+  DBuf<int> bidx;        // tree order -> caller index (Body::IBODY)
+  DBuf<double4> trg4;    // tree order results (pot, gx, gy, gz)
+  DBuf<float4> trgf;     // scratch of the mixed-precision P2P
+  double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
+  bool have_bounds = false, have_tree = false, have_up = false, have_trav = false;
+
+  // Cells (reference Cell order).
+  int ncells = 0;
+  int nlevels = 0;                 // max level + 1
+  DBuf<double4> cX;                // centre + R (w), after the upward pass
+  DBuf<double> cRb;                // R from the builder
+  DBuf<int> cParent, cChild, cNChild, cBody, cNBody, cLevel;
+  DBuf<uint64_t> cKey;
+  DBuf<cplx> M, L;                 // ncells * nterm
+  std::vector<int> lvl_off;        // level-order offsets (host), size nlevels+1
+  DBuf<int> lvl_cells;             // cells sorted by level (stable)
+
+  // Interaction lists (CSR by target cell).
+  int64_t np2p = 0, nm2l = 0, np2p_bodies = 0;
+  DBuf<int> p2p_off, p2p_src, m2l_off, m2l_src;
+  DBuf<unsigned char> p2p_img, m2l_img;
+  DBuf<int> p2p_tgt, m2l_tgt;      // compact target lists
+  int np2p_tgt = 0, nm2l_tgt = 0;
+
+  // M2L lane schedule (a pure function of P).
+  DBuf<uint32_t> m2l_sched;
+  int m2l_sched_P = -1;
+
+  // Scratch.
+  DBuf<unsigned char> tmp;         // hipCUB temporary storage
+  DBuf<uint64_t> k64a, k64b;
+  DBuf<int> ia, ib, ic, id, ie, ifl;
+  DBuf<int4> fa, fb;               // traversal frontiers (ci, cj, img, pad)
+  DBuf<int> cnt;                   // small device counters
+  DBuf<double> red;                // reduction scratch
+
+  exafmm_hip_stats stats = {};
+
+  ~Handle();
+};
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Phase implementations (one translation unit each).
+void run_bounds(Handle &h);
+void run_build_uniform(Handle &h);
+void run_build_adaptive(Handle &h);
+void run_upward(Handle &h);
+void run_lists(Handle &h);
+void run_p2p(Handle &h);
+void run_m2l(Handle &h);
+void run_downward(Handle &h);
+void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
+void run_mass_normalise(Handle &h);
+void run_ropt(Handle &h);
+void run_periodic(Handle &h);
+int64_t count_p2p_body_pairs(Handle &h);
+
+// Scan / sort helpers (hipCUB) in util.hip.
+void exclusive_scan_int(Handle &h, const int *in, int *out, int n);
+void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
+                        bool &result_in_alt);
+void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,
+                        bool &result_in_alt);
+
+}  // namespace exafmm

@@ -1,0 +1,124 @@
+// P2P over the CSR interaction list: one 64-lane workgroup per target leaf.
+//
+// Reference: kernels/LaplaceP2PCPU.cxx:141-169 (pair) and :172-331 (self).
+// Per pair: dX = x_i - x_j - X_periodic, R2 = |dX|^2, skipped when R2 == 0;
+// pot += q_j / r, grad -= q_j dX / r^3.  The reference multiplies by q_i
+// here and divides it out in L2P/normalize (LaplaceSphericalCPU.cxx:235,
+// traversal.h:333-337); the GPU accumulates the normalised quantities
+// directly.
+//
+// Lane layout: a leaf's targets are replicated over G = 64 / T lane groups
+// (T = 16, 32 or 64 targets per group) and group g takes every G-th source
+// body, so short leaves still fill the wavefront.  Source bodies are staged
+// through LDS one source leaf at a time; the G partial sums are combined in a
+// fixed order at the end (deterministic, no atomics).
+#include "internal.h"
+
+namespace exafmm {
+
+__device__ inline void image_shift(int img, double cycle, double &px, double &py, double &pz) {
+  px = (img / 9 - 1) * cycle;
+  py = ((img / 3) % 3 - 1) * cycle;
+  pz = (img % 3 - 1) * cycle;
+}
+
+__global__ void __launch_bounds__(64)
+    k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
+              const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
+              const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
+  __shared__ double4 sb[64];
+  __shared__ double4 sacc[64];
+  const int c = tgt_list[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int tb0 = cBody[c], tn = cNBody[c];
+  const int T = tn <= 16 ? 16 : (tn <= 32 ? 32 : 64);
+  const int G = 64 / T;
+  const int g = lane / T, ti = lane % T;
+  const int e0 = off[c], e1 = off[c + 1];
+  for (int tbase = 0; tbase < tn; tbase += T) {
+    const bool active = tbase + ti < tn;
+    double4 xi = active ? body[tb0 + tbase + ti] : make_double4(0, 0, 0, 0);
+    double pot = 0, ax = 0, ay = 0, az = 0;
+    for (int e = e0; e < e1; e++) {
+      const int s = src[e];
+      double px = 0, py = 0, pz = 0;
+      image_shift(img[e], cycle, px, py, pz);
+      const double xix = xi.x - px, xiy = xi.y - py, xiz = xi.z - pz;
+      const int sb0 = cBody[s], sn = cNBody[s];
+      for (int sbase = 0; sbase < sn; sbase += 64) {
+        const int cnt = min(64, sn - sbase);
+        __syncthreads();
+        if (lane < cnt) sb[lane] = body[sb0 + sbase + lane];
+        __syncthreads();
+        for (int k = g; k < cnt; k += G) {
+          const double4 xj = sb[k];
+          const double dx = xix - xj.x, dy = xiy - xj.y, dz = xiz - xj.z;
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 != 0) {
+            const double invr = rsqrt(r2);
+            const double qinvr = xj.w * invr;
+            const double qinvr3 = qinvr * invr * invr;
+            pot += qinvr;
+            ax += dx * qinvr3;
+            ay += dy * qinvr3;
+            az += dz * qinvr3;
+          }
+        }
+      }
+    }
+    // Combine the G group partials in group order.
+    __syncthreads();
+    sacc[lane] = make_double4(pot, ax, ay, az);
+    __syncthreads();
+    if (g == 0 && active) {
+      double4 s = sacc[ti];
+      for (int gg = 1; gg < G; gg++) {
+        const double4 o = sacc[gg * T + ti];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tbase + ti];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tbase + ti] = tv;
+    }
+  }
+}
+
+// Target-source body pairs of the P2P list (for the interactions/s metric).
+__global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, const int *__restrict__ off,
+                                   const int *__restrict__ src, const int *__restrict__ cNBody,
+                                   unsigned long long *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntl) return;
+  const int c = tgt_list[i];
+  unsigned long long s = 0;
+  for (int e = off[c]; e < off[c + 1]; e++) s += (unsigned long long)cNBody[src[e]];
+  atomicAdd(out, s * (unsigned long long)cNBody[c]);
+}
+
+void run_p2p(Handle &h) {
+  EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), h.stream));
+  if (h.np2p_tgt == 0) return;
+  hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, h.stream, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
+                     h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+  EXA_HIP(hipGetLastError());
+}
+
+int64_t count_p2p_body_pairs(Handle &h) {
+  if (h.np2p_tgt == 0) return 0;
+  h.red.reserve(2);
+  EXA_HIP(hipMemsetAsync(h.red.p, 0, sizeof(unsigned long long), h.stream));
+  hipLaunchKernelGGL(k_count_body_pairs, dim3(div_up(h.np2p_tgt, 256)), dim3(256), 0, h.stream, h.p2p_tgt.p,
+                     h.np2p_tgt, h.p2p_off.p, h.p2p_src.p, h.cNBody.p, (unsigned long long *)h.red.p);
+  unsigned long long v = 0;
+  EXA_HIP(hipMemcpyAsync(&v, h.red.p, sizeof(v), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  return (int64_t)v;
+}
+
+}  // namespace exafmm

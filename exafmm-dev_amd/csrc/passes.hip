@@ -1,0 +1,349 @@
+// Upward pass (P2M, M2M, setRmax, R /= theta) and downward pass (L2L, L2P),
+// level-synchronous: one 64-lane workgroup per cell, one launch per level.
+//
+// Reference: include/up_down_pass.h:45-148 and
+// kernels/LaplaceSphericalCPU.cxx:103-147,193-256.  Lanes own expansion
+// coefficients (P2M, M2M, L2L) or bodies (L2P).  Bodies are summed in body
+// order and children in child order, as the reference does.
+#include <cmath>
+
+#include "harmonics.h"
+#include "internal.h"
+
+namespace exafmm {
+
+__device__ inline void coef_nm(int t, int &n, int &m) {
+  n = 0;
+  while (tri(n + 1) <= t) n++;
+  m = t - tri(n);
+}
+
+// |a - b| with the reference's rounding: norm() sums x^2, y^2, z^2 left to
+// right without contraction (vec.h:163-167), then std::sqrt.
+__device__ inline double exact_dist(double ax, double ay, double az, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  const double dx = ax - bx, dy = ay - by, dz = az - bz;
+  double t = 0;
+  t += dx * dx;
+  t += dy * dy;
+  t += dz * dz;
+  return sqrt(t);
+}
+
+template <int P>
+__global__ void __launch_bounds__(64)
+    k_upward(const int *__restrict__ cells, const double4 *__restrict__ body, double4 *__restrict__ cX,
+             const double *__restrict__ cRb, const int *__restrict__ cChild, const int *__restrict__ cNChild,
+             const int *__restrict__ cBody, const int *__restrict__ cNBody, cplx *__restrict__ M, double theta,
+             int useRmax) {
+  constexpr int NT = Expn<P>::NT;
+  constexpr int NO = (NT + 63) / 64;  // coefficients per lane
+  __shared__ Geom sg[64];
+  __shared__ double sq[64];
+  __shared__ cplx sY[NT];
+  __shared__ cplx sM[NT];
+  __shared__ double sred[64];
+  const int c = cells[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double4 Xc = cX[c];
+  const int nchild = cNChild[c];
+  double acc_re[NO], acc_im[NO];
+  int tn[NO], tm[NO];
+  for (int o = 0; o < NO; o++) {
+    acc_re[o] = acc_im[o] = 0;
+    const int t = lane + 64 * o;
+    coef_nm(t < NT ? t : 0, tn[o], tm[o]);
+  }
+  double rmax = 0;
+  if (nchild == 0) {
+    // P2M: M_n^m += q conj(R_n^m(x_b - X_c)) over the cell's bodies, in order.
+    const int b0 = cBody[c], nb = cNBody[c];
+    for (int base = 0; base < nb; base += 64) {
+      const int cnt = min(64, nb - base);
+      if (lane < cnt) {
+        const double4 v = body[b0 + base + lane];
+        sg[lane] = make_geom(v.x - Xc.x, v.y - Xc.y, v.z - Xc.z);
+        sq[lane] = v.w;
+        const double r = exact_dist(Xc.x, Xc.y, Xc.z, v.x, v.y, v.z);
+        rmax = r > rmax ? r : rmax;
+      }
+      __syncthreads();
+      for (int k = 0; k < cnt; k++) {
+        const Geom g = sg[k];
+        const double q = sq[k];
+#pragma unroll
+        for (int o = 0; o < NO; o++) {
+          if (lane + 64 * o < NT) {
+            double re, im;
+            regular_nm(tn[o], tm[o], g, re, im);
+            acc_re[o] += q * re;
+            acc_im[o] -= q * im;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    // M2M (LaplaceSphericalCPU.cxx:120-147): child multipoles shifted to X_c.
+    const int ch0 = cChild[c];
+    for (int ci = 0; ci < nchild; ci++) {
+      const int cc = ch0 + ci;
+      const double4 Xj = cX[cc];
+      const Geom g = make_geom(Xc.x - Xj.x, Xc.y - Xj.y, Xc.z - Xj.z);
+      for (int t = lane; t < NT; t += 64) {
+        int n, m;
+        coef_nm(t, n, m);
+        double re, im;
+        regular_nm(n, m, g, re, im);
+        sY[t] = {re, im};
+        sM[t] = M[(size_t)cc * NT + t];
+      }
+      if (lane == 0) {
+        const double r = exact_dist(Xc.x, Xc.y, Xc.z, Xj.x, Xj.y, Xj.z) + Xj.w;
+        rmax = r > rmax ? r : rmax;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int o = 0; o < NO; o++) {
+        if (lane + 64 * o >= NT) continue;
+        const int j = tn[o], k = tm[o];
+        double lr = 0, li = 0;
+        for (int n = 0; n <= j; n++) {
+          // m from max(-n, -j+k+n) to min(k-1, n): source (j-n, k-m), Y_n^{-m}
+          const int mlo = max(-n, -j + k + n), mhi = min(k - 1, n);
+          for (int m = mlo; m <= mhi; m++) {
+            const cplx s = sM[tri(j - n) + k - m];
+            cplx y = sY[tri(n) + (m >= 0 ? m : -m)];
+            if (m >= 0) y.im = -y.im;  // Y_n^{-m} = conj(Y_n^m)
+            const double f = ((m < 0 && (m & 1)) ? -1.0 : 1.0) * ((n & 1) ? -1.0 : 1.0);
+            lr += f * (s.re * y.re - s.im * y.im);
+            li += f * (s.re * y.im + s.im * y.re);
+          }
+          // m from k to min(n, j+k-n): conj(source (j-n, m-k)), Y_n^{-m}
+          const int mhi2 = min(n, j + k - n);
+          for (int m = k; m <= mhi2; m++) {
+            const cplx s = sM[tri(j - n) - k + m];
+            const cplx y = sY[tri(n) + m];
+            // conj(s) * conj(y) = conj(s * y)
+            const double f = ((k + n + m) & 1) ? -1.0 : 1.0;
+            lr += f * (s.re * y.re - s.im * y.im);
+            li -= f * (s.re * y.im + s.im * y.re);
+          }
+        }
+        acc_re[o] += lr;
+        acc_im[o] += li;
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NO; o++) {
+    const int t = lane + 64 * o;
+    if (t < NT) M[(size_t)c * NT + t] = {acc_re[o], acc_im[o]};
+  }
+  // setRmax (up_down_pass.h:51-67) and R /= theta (:82).
+  sred[lane] = rmax;
+  __syncthreads();
+  if (lane == 0) {
+    double rm = 0;
+    for (int i = 0; i < 64; i++) rm = sred[i] > rm ? sred[i] : rm;
+    double R = cRb[c];
+    if (useRmax) R = R < rm ? R : rm;
+    R /= theta;
+    cX[c].w = R;
+  }
+}
+
+// L2L (LaplaceSphericalCPU.cxx:193-224) for every cell of one level, then
+// L2P (:226-256) for the leaves among them.
+template <int P>
+__global__ void __launch_bounds__(64)
+    k_downward(const int *__restrict__ cells, const double4 *__restrict__ body, const double4 *__restrict__ cX,
+               const int *__restrict__ cParent, const int *__restrict__ cNChild, const int *__restrict__ cBody,
+               const int *__restrict__ cNBody, const cplx *__restrict__ M, cplx *__restrict__ L,
+               double4 *__restrict__ trg, int do_l2l, int mass) {
+  constexpr int NT = Expn<P>::NT;
+  __shared__ cplx sY[NT];
+  __shared__ cplx sL[NT];
+  __shared__ cplx sLc[NT];
+  const int c = cells[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double4 Xc = cX[c];
+  if (do_l2l) {
+    const int p = cParent[c];
+    const double4 Xp = cX[p];
+    const Geom g = make_geom(Xc.x - Xp.x, Xc.y - Xp.y, Xc.z - Xp.z);
+    for (int t = lane; t < NT; t += 64) {
+      int n, m;
+      coef_nm(t, n, m);
+      double re, im;
+      regular_nm(n, m, g, re, im);
+      sY[t] = {re, im};
+      sL[t] = L[(size_t)p * NT + t];
+    }
+    double m0 = 1;
+    if (mass) m0 = M[(size_t)c * NT].re;  // L /= M[0], M[0] is real
+    __syncthreads();
+    for (int t = lane; t < NT; t += 64) {
+      int j, k;
+      coef_nm(t, j, k);
+      double lr = 0, li = 0;
+      for (int n = j; n < P; n++) {
+        // m from j+k-n to -1: conj(Lp_n^{-m}) Y_{n-j}^{m-k} (-1)^k
+        for (int m = j + k - n; m < 0; m++) {
+          const cplx s = sL[tri(n) - m];
+          const int a = n - j, b = m - k;  // b < 0
+          cplx y = sY[tri(a) - b];
+          y.im = -y.im;
+          const double f = (k & 1) ? -1.0 : 1.0;
+          lr += f * (s.re * y.re + s.im * y.im);
+          li += f * (s.re * y.im - s.im * y.re);
+        }
+        for (int m = 0; m <= n; m++) {
+          const int a = n - j, b = m - k;
+          if (a < (b >= 0 ? b : -b)) continue;
+          const cplx s = sL[tri(n) + m];
+          cplx y = sY[tri(a) + (b >= 0 ? b : -b)];
+          if (b < 0) y.im = -y.im;
+          const double f = (m < k && ((m - k) & 1)) ? -1.0 : 1.0;
+          lr += f * (s.re * y.re - s.im * y.im);
+          li += f * (s.re * y.im + s.im * y.re);
+        }
+      }
+      cplx own = L[(size_t)c * NT + t];
+      if (mass) {
+        own.re /= m0;
+        own.im /= m0;
+      }
+      own.re += lr;
+      own.im += li;
+      L[(size_t)c * NT + t] = own;
+      sLc[t] = own;
+    }
+  } else {
+    for (int t = lane; t < NT; t += 64) sLc[t] = L[(size_t)c * NT + t];
+  }
+  if (cNChild[c] != 0) return;
+  __syncthreads();
+  // L2P: lanes own bodies.  The potential is sum_n sum_m w Re(L_n^m R_n^m)
+  // (w = 1 for m = 0, else 2), as in LaplaceSphericalCPU.cxx:236-250.  The
+  // gradient is taken in Cartesian form from the ladder relations of the
+  // regular harmonics,
+  //   d/dz R_n^m = -R_{n-1}^m,  (d/dx + i d/dy) R_n^m = -R_{n-1}^{m+1},
+  //   (d/dx - i d/dy) R_n^m = R_{n-1}^{m-1}  (m >= 1; -conj R_{n-1}^1 at m = 0),
+  // so every R_n'^m' (n' <= P-2) feeds the degree-(n'+1) terms.  This is the
+  // same function as the reference's spherical-coordinate form without its
+  // 1/r and 1/sin(theta) factors, so it stays finite at the expansion centre
+  // and on the polar axis, where cart2sph/sph2cart give NaN.
+  const int b0 = cBody[c], nb = cNBody[c];
+  for (int bi = lane; bi < nb; bi += 64) {
+    const double4 v = body[b0 + bi];
+    const Geom g = make_geom(v.x - Xc.x, v.y - Xc.y, v.z - Xc.z);
+    const double rho = g.r, x = g.ct, y = g.st;
+    double T0 = 0, gx = 0, gy = 0, gz = 0;
+    double fact = 1, pn = 1, rhom = 1;
+    double er = 1, ei = 0;  // e^{i m beta}
+    auto accumulate = [&](int n, int m, double Rr, double Ri) {
+      const cplx Lv = sLc[tri(n) + m];
+      T0 += (m == 0 ? 1.0 : 2.0) * (Lv.re * Rr - Lv.im * Ri);
+      if (n + 1 >= P) return;
+      const int n1 = n + 1;
+      if (m >= 1) {  // D+ of (n+1, m-1)
+        const cplx A = sLc[tri(n1) + m - 1];
+        const double w = (m == 1) ? 0.5 : 1.0;
+        gx -= w * (A.re * Rr - A.im * Ri);
+        gy -= w * (A.re * Ri + A.im * Rr);
+      }
+      {  // D- of (n+1, m+1)
+        const cplx A = sLc[tri(n1) + m + 1];
+        gx += A.re * Rr - A.im * Ri;
+        gy -= A.re * Ri + A.im * Rr;
+      }
+      if (m == 1) {  // D- of (n+1, 0): -conj(R_n^1)
+        const cplx A = sLc[tri(n1)];
+        gx -= 0.5 * (A.re * Rr + A.im * Ri);
+        gy += 0.5 * (A.im * Rr - A.re * Ri);
+      }
+      {  // d/dz of (n+1, m)
+        const cplx A = sLc[tri(n1) + m];
+        gz -= (m == 0 ? 1.0 : 2.0) * (A.re * Rr - A.im * Ri);
+      }
+    };
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      double p = pn;
+      double p1 = p;
+      p = x * (2 * m + 1) * p1;
+      accumulate(m, m, rhom * p1 * er, rhom * p1 * ei);
+      rhom *= rho;
+      double rhon = rhom;
+#pragma unroll
+      for (int n = m + 1; n < P; n++) {
+        rhon /= -(n + m);
+        accumulate(n, m, rhon * p * er, rhon * p * ei);
+        const double p2 = p1;
+        p1 = p;
+        p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+        rhon *= rho;
+      }
+      rhom /= -(2 * m + 2) * (2 * m + 1);
+      pn = -pn * fact * y;
+      fact += 2;
+      const double t = er * g.cp - ei * g.sp;
+      ei = er * g.sp + ei * g.cp;
+      er = t;
+    }
+    double4 tv = trg[b0 + bi];
+    tv.x += T0;
+    tv.y += gx;
+    tv.z += gy;
+    tv.w += gz;
+    trg[b0 + bi] = tv;
+  }
+}
+
+template <int P>
+static void upward_impl(Handle &h) {
+  for (int t = h.nlevels - 1; t >= 0; t--) {
+    const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t], h.body4.p, h.cX.p,
+                       h.cRb.p, h.cChild.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax);
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+template <int P>
+static void downward_impl(Handle &h) {
+  for (int t = 0; t < h.nlevels; t++) {
+    const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
+    if (cnt == 0) continue;
+    // Level 0 is the root: no L2L; L2P only when the root is a leaf
+    // (up_down_pass.h:139).
+    hipLaunchKernelGGL(k_downward<P>, dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t], h.body4.p,
+                       h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p, t > 0 ? 1 : 0,
+                       h.cfg.mass);
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+#define EXA_DISPATCH_P(fn, h)                                                        \
+  switch ((h).cfg.P) {                                                               \
+    case 4: fn<4>(h); break;                                                         \
+    case 6: fn<6>(h); break;                                                         \
+    case 8: fn<8>(h); break;                                                         \
+    case 10: fn<10>(h); break;                                                       \
+    case 12: fn<12>(h); break;                                                       \
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");    \
+  }
+
+void run_upward(Handle &h) {
+  h.M.resize((size_t)h.ncells * h.nterm);
+  h.L.resize((size_t)h.ncells * h.nterm);
+  EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
+  EXA_DISPATCH_P(upward_impl, h);
+}
+
+void run_downward(Handle &h) { EXA_DISPATCH_P(downward_impl, h); }
+
+}  // namespace exafmm

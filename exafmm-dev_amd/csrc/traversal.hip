@@ -1,0 +1,265 @@
+// Dual tree traversal as a breadth-first frontier of cell pairs, emitting
+// P2P and M2L interaction lists (reference: include/traversal.h:37-124,
+// 188-264).  Compiled with -ffp-contract=off: the MAC must round exactly as
+// the reference does so that both produce the same lists.
+//
+// Each frontier pass classifies every (Ci, Cj, image) pair with the
+// reference's rule -- M2L when |Xi - Xj - Xp|^2 > (Ri + Rj)^2, P2P when both
+// are leaves, otherwise the split of splitCell (:188-212) -- and compacts the
+// three outputs with exclusive scans, so the lists are deterministic.  The
+// non-mutual recursion visits the same set of pairs as the reference's
+// TraverseRange quadrant order; only the order differs, and the lists are
+// regrouped by target cell (CSR) afterwards.
+#include "internal.h"
+
+namespace exafmm {
+
+struct CellView {
+  const double4 *X;
+  const int *child, *nchild, *nbody;
+};
+
+__device__ inline void image_offset(int img, double cycle, double &px, double &py, double &pz) {
+  px = (img / 9 - 1) * cycle;
+  py = ((img / 3) % 3 - 1) * cycle;
+  pz = (img % 3 - 1) * cycle;
+}
+
+// action: 0 M2L, 1 P2P, 2 split Ci, 3 split Cj, 4 split both.
+__device__ inline int classify(const CellView &cv, int ci, int cj, int img, double cycle, int nspawn) {
+#pragma clang fp contract(off)
+  const double4 Xi = cv.X[ci], Xj = cv.X[cj];
+  double px, py, pz;
+  image_offset(img, cycle, px, py, pz);
+  const double dx = Xi.x - Xj.x - px;
+  const double dy = Xi.y - Xj.y - py;
+  const double dz = Xi.z - Xj.z - pz;
+  double R2 = 0;
+  R2 += dx * dx;
+  R2 += dy * dy;
+  R2 += dz * dz;
+  const double Rs = Xi.w + Xj.w;
+  if (R2 > Rs * Rs) return 0;
+  const int li = cv.nchild[ci] == 0, lj = cv.nchild[cj] == 0;
+  if (li && lj) return cv.nbody[cj] == 0 ? 0 : 1;
+  if (lj) return 2;
+  if (li) return 3;
+  if (cv.nbody[ci] + cv.nbody[cj] >= nspawn) return 4;
+  return Xi.w >= Xj.w ? 2 : 3;
+}
+
+__global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn,
+                           int *__restrict__ act, int *__restrict__ c_m2l, int *__restrict__ c_p2p,
+                           int *__restrict__ c_next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nf) return;
+  if (i == nf) {  // trailing zero so that the exclusive scans end in the totals
+    c_m2l[i] = c_p2p[i] = c_next[i] = 0;
+    return;
+  }
+  const int4 f = fr[i];
+  const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn);
+  act[i] = a;
+  c_m2l[i] = a == 0;
+  c_p2p[i] = a == 1;
+  int nx = 0;
+  if (a == 2)
+    nx = cv.nchild[f.x];
+  else if (a == 3)
+    nx = cv.nchild[f.y];
+  else if (a == 4)
+    nx = cv.nchild[f.x] * cv.nchild[f.y];
+  c_next[i] = nx;
+}
+
+__global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, const int *__restrict__ act,
+                       const int *__restrict__ o_m2l, const int *__restrict__ o_p2p, const int *__restrict__ o_next,
+                       int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                       unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
+                       unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int4 f = fr[i];
+  const int a = act[i];
+  if (a == 0) {
+    const int64_t o = base_m2l + o_m2l[i];
+    m2l_i[o] = f.x;
+    m2l_j[o] = f.y;
+    m2l_img[o] = (unsigned char)f.z;
+  } else if (a == 1) {
+    const int64_t o = base_p2p + o_p2p[i];
+    p2p_i[o] = f.x;
+    p2p_j[o] = f.y;
+    p2p_img[o] = (unsigned char)f.z;
+  } else {
+    int o = o_next[i];
+    if (a == 2) {
+      const int c0 = cv.child[f.x], nc = cv.nchild[f.x];
+      for (int c = 0; c < nc; c++) next[o++] = make_int4(c0 + c, f.y, f.z, 0);
+    } else if (a == 3) {
+      const int c0 = cv.child[f.y], nc = cv.nchild[f.y];
+      for (int c = 0; c < nc; c++) next[o++] = make_int4(f.x, c0 + c, f.z, 0);
+    } else {
+      const int ci0 = cv.child[f.x], nci = cv.nchild[f.x];
+      const int cj0 = cv.child[f.y], ncj = cv.nchild[f.y];
+      for (int a1 = 0; a1 < nci; a1++)
+        for (int b1 = 0; b1 < ncj; b1++) next[o++] = make_int4(ci0 + a1, cj0 + b1, f.z, 0);
+    }
+  }
+}
+
+__global__ void k_init_frontier(int4 *__restrict__ fr, int images) {
+  const int i = threadIdx.x;
+  if (images == 0) {
+    if (i == 0) fr[0] = make_int4(0, 0, 13, 0);
+  } else if (i < 27) {
+    fr[i] = make_int4(0, 0, i, 0);
+  }
+}
+
+// Histogram of targets (counts are order-independent, hence deterministic).
+__global__ void k_count_targets(const int *__restrict__ tgt, int64_t n, int *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[tgt[i]], 1);
+}
+
+__global__ void k_permute_pairs(const int *__restrict__ order, int64_t n, const int *__restrict__ src_in,
+                                const unsigned char *__restrict__ img_in, int *__restrict__ src_out,
+                                unsigned char *__restrict__ img_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = order[i];
+  src_out[i] = src_in[s];
+  img_out[i] = img_in[s];
+}
+
+__global__ void k_flag_nonzero(const int *__restrict__ cnt, int n, int *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) flag[i] = i < n ? (cnt[i] > 0) : 0;
+}
+
+__global__ void k_compact(const int *__restrict__ flag, const int *__restrict__ scan, int n, int *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) out[scan[i]] = i;
+}
+
+__global__ void k_seq(int *__restrict__ a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (int)i;
+}
+
+// Grow a buffer, keeping its first `keep` elements.
+template <typename T>
+static void grow_keep(Handle &h, DBuf<T> &b, size_t want, size_t keep) {
+  if (want <= b.cap) return;
+  DBuf<T> nb;
+  nb.reserve(want * 2);
+  if (keep) EXA_HIP(hipMemcpyAsync(nb.p, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  b.release();
+  b.p = nb.p;
+  b.cap = nb.cap;
+  nb.p = nullptr;
+  nb.cap = 0;
+}
+
+// Group (tgt[k], src[k], img[k]) by target: CSR offsets off[0..ncells], the
+// sources and images in target order (stable w.r.t. emission order), and the
+// compact list of targets with at least one entry.
+static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<unsigned char> &img,
+                            DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist,
+                            int &ntl) {
+  const int nc = h.ncells;
+  const int T = 256;
+  off.resize(nc + 1);
+  src_out.resize(np);
+  img_out.resize(np);
+  h.ia.reserve(np + 1);
+  h.ib.reserve(np + 1);
+  h.ic.reserve(np + 1);
+  h.ifl.reserve(nc + 1);
+  EXA_HIP(hipMemsetAsync(h.ifl.p, 0, (nc + 1) * sizeof(int), h.stream));
+  if (np > 0) {
+    hipLaunchKernelGGL(k_count_targets, dim3(div_up(np, T)), dim3(T), 0, h.stream, tgt.p, np, h.ifl.p);
+    hipLaunchKernelGGL(k_seq, dim3(div_up(np, T)), dim3(T), 0, h.stream, h.ia.p, np);
+    int bits = 1;
+    while ((1 << bits) < nc) bits++;
+    bool alt = false;
+    // keys: a copy of tgt (sorted in place), values: emission index.
+    EXA_HIP(hipMemcpyAsync(h.ic.p, tgt.p, np * sizeof(int), hipMemcpyDeviceToDevice, h.stream));
+    h.id.reserve(np + 1);
+    sort_pairs_int_int(h, h.ic.p, h.id.p, h.ia.p, h.ib.p, (int)np, bits, alt);
+    const int *order = alt ? h.ib.p : h.ia.p;
+    hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, h.stream, order, np, src.p, img.p, src_out.p,
+                       img_out.p);
+  }
+  exclusive_scan_int(h, h.ifl.p, off.p, nc + 1);
+  // Compact non-empty targets.
+  h.ie.reserve(2 * (size_t)nc + 2);
+  int *flag = h.ie.p, *scan = h.ie.p + nc + 1;
+  hipLaunchKernelGGL(k_flag_nonzero, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, h.ifl.p, nc, flag);
+  exclusive_scan_int(h, flag, scan, nc + 1);
+  tlist.resize(nc);
+  hipLaunchKernelGGL(k_compact, dim3(div_up(nc, T)), dim3(T), 0, h.stream, flag, scan, nc, tlist.p);
+  EXA_HIP(hipMemcpyAsync(&ntl, scan + nc, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+}
+
+void run_lists(Handle &h) {
+  const int T = 256;
+  CellView cv{h.cX.p, h.cChild.p, h.cNChild.p, h.cNBody.p};
+  // Raw (emission-order) lists.
+  DBuf<int> mi, mj, pi, pj;
+  DBuf<unsigned char> mimg, pimg;
+  int64_t nm = 0, np = 0;
+  int nf = h.cfg.images == 0 ? 1 : 27;
+  h.fa.reserve(64);
+  hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
+  int64_t body_pairs = 0;
+  (void)body_pairs;
+  while (nf > 0) {
+    h.ia.reserve(nf + 1);  // action
+    h.ib.reserve(3 * ((size_t)nf + 1));  // counts
+    h.ic.reserve(3 * ((size_t)nf + 1));  // scans
+    int *act = h.ia.p;
+    int *cm = h.ib.p, *cp = h.ib.p + (nf + 1), *cn = h.ib.p + 2 * (nf + 1);
+    int *om = h.ic.p, *op = h.ic.p + (nf + 1), *on = h.ic.p + 2 * (nf + 1);
+    hipLaunchKernelGGL(k_classify, dim3(div_up(nf + 1, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
+                       h.cfg.nspawn, act, cm, cp, cn);
+    exclusive_scan_int(h, cm, om, nf + 1);
+    exclusive_scan_int(h, cp, op, nf + 1);
+    exclusive_scan_int(h, cn, on, nf + 1);
+    int tot[3];
+    EXA_HIP(hipMemcpyAsync(&tot[0], om + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipMemcpyAsync(&tot[1], op + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipMemcpyAsync(&tot[2], on + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    grow_keep(h, mi, nm + tot[0] + 1, nm);
+    grow_keep(h, mj, nm + tot[0] + 1, nm);
+    grow_keep(h, mimg, nm + tot[0] + 1, nm);
+    grow_keep(h, pi, np + tot[1] + 1, np);
+    grow_keep(h, pj, np + tot[1] + 1, np);
+    grow_keep(h, pimg, np + tot[1] + 1, np);
+    h.fb.reserve((size_t)tot[2] + 1);
+    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, act, om, op, on, nm, np, mi.p,
+                       mj.p, mimg.p, pi.p, pj.p, pimg.p, h.fb.p);
+    nm += tot[0];
+    np += tot[1];
+    nf = tot[2];
+    std::swap(h.fa.p, h.fb.p);
+    std::swap(h.fa.cap, h.fb.cap);
+  }
+  h.nm2l = nm;
+  h.np2p = np;
+  group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt);
+  group_by_target(h, np, pi, pj, pimg, h.p2p_off, h.p2p_src, h.p2p_img, h.p2p_tgt, h.np2p_tgt);
+  mi.release();
+  mj.release();
+  pi.release();
+  pj.release();
+  mimg.release();
+  pimg.release();
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exafmm

@@ -1,0 +1,310 @@
+// Bounds and the uniform Morton-key tree (reference: include/bound_box.h and
+// include/build_tree_omp.h).  Compiled with -ffp-contract=off: keys, cell
+// centres and radii must round exactly as the reference does on x86-64.
+//
+// Layout produced (identical to build_tree_omp.h after reverseOrder,
+// :200-211): cells are level-contiguous, root first; inside a level they run
+// in descending Morton order; bodies run in ascending Morton order (stable).
+#include <cmath>
+
+#include "internal.h"
+
+namespace exafmm {
+
+// ---------------------------------------------------------------------------
+// Bounds: grid-stride min/max, one partial per block, then one block.
+constexpr int kBoundsBlocks = 512;
+
+__global__ void __launch_bounds__(256) k_bounds_partial(const double4 *__restrict__ b, int n, double *__restrict__ part) {
+  __shared__ double s[6][256];
+  double mn[3] = {b[0].x, b[0].y, b[0].z};
+  double mx[3] = {mn[0], mn[1], mn[2]};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double4 v = b[i];
+    mn[0] = v.x < mn[0] ? v.x : mn[0];
+    mn[1] = v.y < mn[1] ? v.y : mn[1];
+    mn[2] = v.z < mn[2] ? v.z : mn[2];
+    mx[0] = v.x > mx[0] ? v.x : mx[0];
+    mx[1] = v.y > mx[1] ? v.y : mx[1];
+    mx[2] = v.z > mx[2] ? v.z : mx[2];
+  }
+  for (int d = 0; d < 3; d++) {
+    s[d][threadIdx.x] = mn[d];
+    s[3 + d][threadIdx.x] = mx[d];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      for (int d = 0; d < 3; d++) {
+        const double a = s[d][threadIdx.x + w];
+        s[d][threadIdx.x] = a < s[d][threadIdx.x] ? a : s[d][threadIdx.x];
+        const double c = s[3 + d][threadIdx.x + w];
+        s[3 + d][threadIdx.x] = c > s[3 + d][threadIdx.x] ? c : s[3 + d][threadIdx.x];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    double r[6];
+    for (int d = 0; d < 6; d++) r[d] = part[d];
+    for (int i = 1; i < nb; i++)
+      for (int d = 0; d < 3; d++) {
+        const double a = part[6 * i + d], c = part[6 * i + 3 + d];
+        r[d] = a < r[d] ? a : r[d];
+        r[3 + d] = c > r[3 + d] ? c : r[3 + d];
+      }
+    for (int d = 0; d < 6; d++) out[d] = r[d];
+  }
+}
+
+void run_bounds(Handle &h) {
+  if (h.n == 0) {
+    for (int d = 0; d < 3; d++) h.xmin[d] = h.xmax[d] = 0;
+    h.have_bounds = true;
+    return;
+  }
+  const int nb = std::min(kBoundsBlocks, div_up(h.n, 256));
+  h.red.reserve(6 * nb + 6);
+  hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, h.stream, h.in4.p, h.n, h.red.p);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, h.stream, h.red.p, nb, h.red.p + 6 * nb);
+  double r[6];
+  EXA_HIP(hipMemcpyAsync(r, h.red.p + 6 * nb, sizeof(r), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  for (int d = 0; d < 3; d++) {
+    h.xmin[d] = r[d];
+    h.xmax[d] = r[3 + d];
+  }
+  h.have_bounds = true;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform builder kernels.
+
+// getKey (build_tree_omp.h:36-57): float cell size promoted to double for
+// the division, truncation to int, bit interleave in int.
+__global__ void k_morton(const double4 *__restrict__ in, int n, double x0, double y0, double z0, float d, int level,
+                         uint64_t *__restrict__ key, int *__restrict__ idx) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const double4 v = in[b];
+  const double dd = (double)d;
+  int ix = (int)((v.x - x0) / dd);
+  int iy = (int)((v.y - y0) / dd);
+  int iz = (int)((v.z - z0) / dd);
+  int id = 0;
+  for (int l = 0; l != level; ++l) {
+    id += (ix & 1) << (3 * l);
+    id += (iy & 1) << (3 * l + 1);
+    id += (iz & 1) << (3 * l + 2);
+    ix >>= 1;
+    iy >>= 1;
+    iz >>= 1;
+  }
+  key[b] = (uint64_t)(int64_t)id;
+  idx[b] = b;
+}
+
+__global__ void k_gather_bodies(const double4 *__restrict__ in, const int *__restrict__ order, int n,
+                                double4 *__restrict__ out, int *__restrict__ bidx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = order[i];
+  out[i] = in[s];
+  bidx[i] = s;
+}
+
+// Head flags of tree level t: body i starts a level-t cell when its key
+// prefix (key >> shift) differs from body i-1's.  flags has n+1 entries; the
+// last is 0 so that the exclusive scan yields the total at index n.
+__global__ void k_level_heads(const uint64_t *__restrict__ key, int n, int shift, int *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int f = 0;
+  if (i < n) f = (i == 0) || ((key[i] >> shift) != (key[i - 1] >> shift));
+  flags[i] = f;
+}
+
+__global__ void k_level_positions(const int *__restrict__ flags, const int *__restrict__ scan, int n,
+                                  int *__restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) pos[scan[i]] = i;
+}
+
+struct LevelArgs {
+  const int *pos;      // heads of this level, ascending
+  const int *scan;     // exclusive scan of this level's flags (n+1)
+  const int *scan_up;  // scan of level t-1 (or nullptr at the root)
+  const int *scan_dn;  // scan of level t+1 (or nullptr at the leaves)
+  int count, count_up, count_dn;
+  int off, off_up, off_dn;
+  int shift;           // 3 * (L - t)
+  float d;             // cell size of this level
+  int leaf;            // 1 at the finest level
+  int level;
+};
+
+// bodies2leafs / leafs2cells (build_tree_omp.h:130-198) fused with the index
+// arithmetic of reverseOrder: ascending rank r at level t lands at
+// off_t + count_t - 1 - r.
+__global__ void k_level_cells(LevelArgs a, const double4 *__restrict__ body, const uint64_t *__restrict__ key, int n,
+                              int nc, double x0, double y0, double z0, double4 *__restrict__ cX,
+                              double *__restrict__ cRb, int *__restrict__ cParent, int *__restrict__ cChild,
+                              int *__restrict__ cNChild, int *__restrict__ cBody, int *__restrict__ cNBody,
+                              uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.count) return;
+  const int start = a.pos[r];
+  const int end = (r + 1 < a.count) ? a.pos[r + 1] : n;
+  const int c = a.off + a.count - 1 - r;
+  const double4 v = body[start];
+  const double dd = (double)a.d;
+  const int ix = (int)((v.x - x0) / dd);
+  const int iy = (int)((v.y - y0) / dd);
+  const int iz = (int)((v.z - z0) / dd);
+  double4 X;
+  X.x = dd * (ix + .5) + x0;
+  X.y = dd * (iy + .5) + y0;
+  X.z = dd * (iz + .5) + z0;
+  X.w = dd * .5;
+  cX[c] = X;
+  cRb[c] = X.w;
+  cBody[c] = start;
+  cNBody[c] = end - start;
+  cKey[c] = (uint64_t)(int64_t)(int)(key[start] >> a.shift);
+  cLevel[c] = a.level;
+  if (a.leaf) {
+    cNChild[c] = 0;
+    cChild[c] = nc;  // reverseOrder maps ICHILD = 0 of a leaf to numCells
+  } else {
+    const int lo = a.scan_dn[start], hi = a.scan_dn[end];
+    cNChild[c] = hi - lo;
+    cChild[c] = a.off_dn + a.count_dn - hi;
+  }
+  if (a.level == 0) {
+    cParent[c] = nc - 1;  // reverseOrder maps the root's IPARENT = 0 to numCells-1
+  } else {
+    const int rp = a.scan_up[start + 1] - 1;
+    cParent[c] = a.off_up + a.count_up - 1 - rp;
+  }
+}
+
+__global__ void k_iota(int *__restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+void run_build_uniform(Handle &h) {
+  const int n = h.n;
+  const int ncrit = h.cfg.ncrit;
+  // build_tree_omp.h:218 (integer n / ncrit inside the log).
+  const int L = n >= ncrit ? 1 + int(std::log((double)(n / ncrit)) / M_LN2 / 3) : 0;
+  if (L > 10) throw Error(EXAFMM_HIP_EUNSUP, "uniform tree deeper than 10 levels overflows the reference's int keys");
+  // bounds2box (build_tree_omp.h:21-33).
+  double Xc[3], R0 = 0;
+  for (int d = 0; d < 3; d++) Xc[d] = (h.xmax[d] + h.xmin[d]) / 2;
+  for (int d = 0; d < 3; d++) {
+    R0 = std::max(Xc[d] - h.xmin[d], R0);
+    R0 = std::max(h.xmax[d] - Xc[d], R0);
+  }
+  R0 *= 1.00001;
+  const float d = 2 * R0 / (1 << L);
+
+  h.k64a.reserve(n + 1);
+  h.k64b.reserve(n + 1);
+  h.ia.reserve(n + 1);
+  h.ib.reserve(n + 1);
+  h.body4.resize(n);
+  h.bidx.resize(n);
+  h.trg4.resize(n);
+  const int T = 256;
+  hipLaunchKernelGGL(k_morton, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.xmin[0], h.xmin[1], h.xmin[2], d,
+                     L, h.k64a.p, h.ia.p);
+  bool alt = false;
+  sort_pairs_u64_int(h, h.k64a.p, h.k64b.p, h.ia.p, h.ib.p, n, 3 * L, alt);
+  uint64_t *key = alt ? h.k64b.p : h.k64a.p;
+  int *order = alt ? h.ib.p : h.ia.p;
+  hipLaunchKernelGGL(k_gather_bodies, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, order, n, h.body4.p, h.bidx.p);
+
+  // Per-level heads and scans: (L+1) x (n+1) ints each.
+  const size_t stride = (size_t)n + 1;
+  h.ic.reserve(stride * (L + 1));  // flags
+  h.id.reserve(stride * (L + 1));  // scans
+  std::vector<int> counts(L + 1);
+  std::vector<int> tot(L + 1);
+  for (int t = 0; t <= L; t++) {
+    int *fl = h.ic.p + stride * t;
+    int *sc = h.id.p + stride * t;
+    hipLaunchKernelGGL(k_level_heads, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, key, n, 3 * (L - t), fl);
+    exclusive_scan_int(h, fl, sc, n + 1);
+  }
+  // Totals: scan[n] of each level.
+  h.cnt.reserve(L + 1);
+  for (int t = 0; t <= L; t++)
+    EXA_HIP(hipMemcpyAsync(&tot[t], h.id.p + stride * t + n, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  int nc = 0;
+  std::vector<int> off(L + 2, 0);
+  for (int t = 0; t <= L; t++) {
+    counts[t] = tot[t];
+    off[t] = nc;
+    nc += counts[t];
+  }
+  off[L + 1] = nc;
+  h.ncells = nc;
+  h.nlevels = L + 1;
+  h.ie.reserve(nc + 1);  // positions, level after level
+  std::vector<int> posoff(L + 1);
+  {
+    int o = 0;
+    for (int t = 0; t <= L; t++) {
+      posoff[t] = o;
+      o += counts[t];
+    }
+  }
+  for (int t = 0; t <= L; t++)
+    hipLaunchKernelGGL(k_level_positions, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ic.p + stride * t,
+                       h.id.p + stride * t, n, h.ie.p + posoff[t]);
+  h.cX.resize(nc);
+  h.cRb.resize(nc);
+  h.cParent.resize(nc);
+  h.cChild.resize(nc);
+  h.cNChild.resize(nc);
+  h.cBody.resize(nc);
+  h.cNBody.resize(nc);
+  h.cKey.resize(nc);
+  h.cLevel.resize(nc);
+  for (int t = 0; t <= L; t++) {
+    LevelArgs a;
+    a.pos = h.ie.p + posoff[t];
+    a.scan = h.id.p + stride * t;
+    a.scan_up = t > 0 ? h.id.p + stride * (t - 1) : nullptr;
+    a.scan_dn = t < L ? h.id.p + stride * (t + 1) : nullptr;
+    a.count = counts[t];
+    a.count_up = t > 0 ? counts[t - 1] : 0;
+    a.count_dn = t < L ? counts[t + 1] : 0;
+    a.off = off[t];
+    a.off_up = t > 0 ? off[t - 1] : 0;
+    a.off_dn = t < L ? off[t + 1] : 0;
+    a.shift = 3 * (L - t);
+    float dt = d;
+    for (int l = 0; l < L - t; l++) dt *= 2;  // leafs2cells doubles d per level (:166)
+    a.d = dt;
+    a.leaf = (t == L);
+    a.level = t;
+    hipLaunchKernelGGL(k_level_cells, dim3(div_up(counts[t], T)), dim3(T), 0, h.stream, a, h.body4.p, key, n, nc,
+                       h.xmin[0], h.xmin[1], h.xmin[2], h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p,
+                       h.cBody.p, h.cNBody.p, h.cKey.p, h.cLevel.p);
+  }
+  // Level order: the layout is already level-contiguous.
+  h.lvl_off.assign(off.begin(), off.begin() + L + 2);
+  h.lvl_cells.resize(nc);
+  hipLaunchKernelGGL(k_iota, dim3(div_up(nc, T)), dim3(T), 0, h.stream, h.lvl_cells.p, nc);
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exafmm

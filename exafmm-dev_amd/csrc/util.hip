@@ -1,0 +1,84 @@
+// Scan and sort helpers over hipCUB (rocPRIM) device primitives, plus the
+// handle destructor.  Sorting and scanning are bookkeeping around the hot
+// path (tree build and list grouping); the hot kernels are hand-written.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+
+namespace exafmm {
+
+Handle::~Handle() {
+  in4.release();
+  body4.release();
+  bidx.release();
+  trg4.release();
+  trgf.release();
+  cX.release();
+  cRb.release();
+  cParent.release();
+  cChild.release();
+  cNChild.release();
+  cBody.release();
+  cNBody.release();
+  cLevel.release();
+  cKey.release();
+  M.release();
+  L.release();
+  lvl_cells.release();
+  p2p_off.release();
+  p2p_src.release();
+  m2l_off.release();
+  m2l_src.release();
+  p2p_img.release();
+  m2l_img.release();
+  p2p_tgt.release();
+  m2l_tgt.release();
+  m2l_sched.release();
+  tmp.release();
+  k64a.release();
+  k64b.release();
+  ia.release();
+  ib.release();
+  ic.release();
+  id.release();
+  ie.release();
+  ifl.release();
+  fa.release();
+  fb.release();
+  cnt.release();
+  red.release();
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void exclusive_scan_int(Handle &h, const int *in, int *out, int n) {
+  size_t bytes = 0;
+  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, h.stream));
+  h.tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(h.tmp.p, bytes, in, out, n, h.stream));
+}
+
+void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
+                        bool &result_in_alt) {
+  hipcub::DoubleBuffer<uint64_t> k(keys, keys_alt);
+  hipcub::DoubleBuffer<int> v(vals, vals_alt);
+  size_t bytes = 0;
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
+  h.tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(h.tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
+  result_in_alt = k.Current() != keys;
+}
+
+void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,
+                        bool &result_in_alt) {
+  hipcub::DoubleBuffer<int> k(keys, keys_alt);
+  hipcub::DoubleBuffer<int> v(vals, vals_alt);
+  size_t bytes = 0;
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
+  h.tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(h.tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
+  result_in_alt = k.Current() != keys;
+}
+
+}  // namespace exafmm

@@ -1,0 +1,233 @@
+"""Python binding of libexafmm_hip (the C-ABI of include/exafmm_hip.h).
+
+`FMM` mirrors the pass-level API of the reference driver examples/serial.cxx:
+get_bounds (BoundBox::getBounds), build_tree (BuildTree::buildTree),
+upward_pass / downward_pass (UpDownPass), traverse
+(Traversal::dualTreeTraversal), direct (Traversal::direct + normalize), and
+evaluate for the whole sequence.  Every call goes to the HIP library; there is
+no CPU fallback: a missing library or a failing call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexafmm_hip.so")
+
+# Every symbol declared in include/exafmm_hip.h.
+EXPORTS = (
+    "exafmm_hip_create", "exafmm_hip_destroy", "exafmm_hip_last_error", "exafmm_hip_set_bodies",
+    "exafmm_hip_set_bodies_device", "exafmm_hip_get_bounds", "exafmm_hip_build_tree", "exafmm_hip_upward_pass",
+    "exafmm_hip_traverse", "exafmm_hip_downward_pass", "exafmm_hip_evaluate", "exafmm_hip_get_cells",
+    "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
+    "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
+)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int),
+        ("theta", ctypes.c_double),
+        ("ncrit", ctypes.c_int),
+        ("nspawn", ctypes.c_int),
+        ("useRmax", ctypes.c_int),
+        ("useRopt", ctypes.c_int),
+        ("mutual", ctypes.c_int),
+        ("images", ctypes.c_int),
+        ("cycle", ctypes.c_double),
+        ("builder", ctypes.c_int),
+        ("mass", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_downward", "t_total")] + [
+        ("num_p2p_pairs", ctypes.c_int64), ("num_m2l_pairs", ctypes.c_int64), ("num_p2p_bodies", ctypes.c_int64),
+        ("num_cells", ctypes.c_int), ("num_levels", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ExafmmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libexafmm_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ExafmmError(f"{LIB_PATH} is missing: build it with `make -C exafmm-dev_amd`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    L.exafmm_hip_create.argtypes = [P(Config), P(vp)]
+    L.exafmm_hip_destroy.argtypes = [vp]
+    L.exafmm_hip_last_error.argtypes = [vp]
+    L.exafmm_hip_last_error.restype = ctypes.c_char_p
+    L.exafmm_hip_set_bodies.argtypes = [vp, ip, vp]
+    L.exafmm_hip_set_bodies_device.argtypes = [vp, ip, vp]
+    L.exafmm_hip_get_bounds.argtypes = [vp, vp, vp]
+    L.exafmm_hip_build_tree.argtypes = [vp, vp, vp, P(ip)]
+    for f in ("upward_pass", "traverse", "downward_pass", "evaluate", "synchronize"):
+        getattr(L, "exafmm_hip_" + f).argtypes = [vp]
+    L.exafmm_hip_get_cells.argtypes = [vp] + [vp] * 6
+    L.exafmm_hip_get_bodies.argtypes = [vp, vp, vp, vp]
+    L.exafmm_hip_get_results.argtypes = [vp, vp]
+    L.exafmm_hip_results_device.argtypes = [vp, P(vp)]
+    L.exafmm_hip_get_list_counts.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64)]
+    L.exafmm_hip_get_lists.argtypes = [vp, vp, vp, vp, vp]
+    L.exafmm_hip_direct.argtypes = [vp, ip, vp, vp]
+    L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
+    for f in EXPORTS:
+        if f != "exafmm_hip_last_error":
+            getattr(L, f).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class FMM:
+    """One particle system on one GPU (one handle, one HIP stream)."""
+
+    def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, mutual=0, images=0,
+                 cycle=2 * np.pi, builder="uniform", mass=False, precision="fp64", device=0):
+        self.lib = load()
+        b = {"uniform": 0, "adaptive": 1}[builder] if isinstance(builder, str) else int(builder)
+        prec = {"fp64": 0, "mixed": 1}[precision] if isinstance(precision, str) else int(precision)
+        self.cfg = Config(P, theta, ncrit, nspawn, useRmax, useRopt, mutual, images, cycle, b, int(mass), prec, device)
+        h = ctypes.c_void_p()
+        rc = self.lib.exafmm_hip_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if rc != 0:
+            raise ExafmmError(f"exafmm_hip_create failed with status {rc}")
+        self.h = h
+        self.P = P
+        self.nterm = P * (P + 1) // 2
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.exafmm_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.exafmm_hip_last_error(self.h)
+            raise ExafmmError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    # -- bodies ------------------------------------------------------------
+    def set_bodies(self, xyzq):
+        a = np.ascontiguousarray(xyzq, dtype=np.float64)
+        if a.ndim != 2 or a.shape[1] != 4:
+            raise ValueError("bodies must be an (n, 4) array of x, y, z, q")
+        self.n = a.shape[0]
+        self._check(self.lib.exafmm_hip_set_bodies(self.h, self.n, _ptr(a)), "set_bodies")
+
+    def set_bodies_device(self, n, dev_ptr):
+        self.n = int(n)
+        self._check(self.lib.exafmm_hip_set_bodies_device(self.h, self.n, ctypes.c_void_p(dev_ptr)),
+                    "set_bodies_device")
+
+    # -- passes ------------------------------------------------------------
+    def get_bounds(self):
+        a, b = np.zeros(3), np.zeros(3)
+        self._check(self.lib.exafmm_hip_get_bounds(self.h, _ptr(a), _ptr(b)), "get_bounds")
+        return a, b
+
+    def build_tree(self, bounds=None):
+        nc = ctypes.c_int()
+        if bounds is None:
+            rc = self.lib.exafmm_hip_build_tree(self.h, None, None, ctypes.byref(nc))
+        else:
+            a = np.ascontiguousarray(bounds[0], np.float64)
+            b = np.ascontiguousarray(bounds[1], np.float64)
+            rc = self.lib.exafmm_hip_build_tree(self.h, _ptr(a), _ptr(b), ctypes.byref(nc))
+        self._check(rc, "build_tree")
+        return nc.value
+
+    def upward_pass(self):
+        self._check(self.lib.exafmm_hip_upward_pass(self.h), "upward_pass")
+
+    def traverse(self):
+        self._check(self.lib.exafmm_hip_traverse(self.h), "traverse")
+
+    def downward_pass(self):
+        self._check(self.lib.exafmm_hip_downward_pass(self.h), "downward_pass")
+
+    def evaluate(self):
+        self._check(self.lib.exafmm_hip_evaluate(self.h), "evaluate")
+
+    def synchronize(self):
+        self._check(self.lib.exafmm_hip_synchronize(self.h), "synchronize")
+
+    # -- readback ----------------------------------------------------------
+    def cells(self):
+        s = self.stats()
+        nc = s["num_cells"]
+        ints = np.zeros((nc, 5), np.int32)
+        icell = np.zeros(nc, np.uint64)
+        X = np.zeros((nc, 3))
+        R = np.zeros(nc)
+        M = np.zeros((nc, self.nterm), np.complex128)
+        L = np.zeros((nc, self.nterm), np.complex128)
+        self._check(self.lib.exafmm_hip_get_cells(self.h, _ptr(ints), _ptr(icell), _ptr(X), _ptr(R), _ptr(M),
+                                                  _ptr(L)), "get_cells")
+        return dict(iparent=ints[:, 0].copy(), ichild=ints[:, 1].copy(), nchild=ints[:, 2].copy(),
+                    ibody=ints[:, 3].copy(), nbody=ints[:, 4].copy(), icell=icell, X=X, R=R, M=M, L=L)
+
+    def bodies(self):
+        xyzq = np.zeros((self.n, 4))
+        ib = np.zeros(self.n, np.int32)
+        trg = np.zeros((self.n, 4))
+        self._check(self.lib.exafmm_hip_get_bodies(self.h, _ptr(xyzq), _ptr(ib), _ptr(trg)), "get_bodies")
+        return dict(xyzq=xyzq, ibody=ib, trg=trg)
+
+    def results(self):
+        """(n, 4) potential and gradient in the caller's body order."""
+        out = np.zeros((self.n, 4))
+        self._check(self.lib.exafmm_hip_get_results(self.h, _ptr(out)), "get_results")
+        return out
+
+    def lists(self):
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        self._check(self.lib.exafmm_hip_get_list_counts(self.h, ctypes.byref(a), ctypes.byref(b)), "list_counts")
+        pi = np.zeros(a.value, np.int32)
+        pj = np.zeros(a.value, np.int32)
+        mi = np.zeros(b.value, np.int32)
+        mj = np.zeros(b.value, np.int32)
+        self._check(self.lib.exafmm_hip_get_lists(self.h, _ptr(pi), _ptr(pj), _ptr(mi), _ptr(mj)), "get_lists")
+        return dict(p2p_i=pi, p2p_j=pj, m2l_i=mi, m2l_j=mj)
+
+    def direct(self, target_ibody):
+        t = np.ascontiguousarray(target_ibody, np.int32)
+        out = np.zeros((t.shape[0], 4))
+        self._check(self.lib.exafmm_hip_direct(self.h, t.shape[0], _ptr(t), _ptr(out)), "direct")
+        return out
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.exafmm_hip_get_stats(self.h, ctypes.byref(s)), "get_stats")
+        return s.as_dict()
+
+
+def rel_l2(a, b):
+    """Relative L2 difference sqrt(sum (a-b)^2 / sum b^2) (include/verify.h)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))

@@ -1,0 +1,129 @@
+/* exafmm_hip.h -- C-ABI of the MI355X-native ExaFMM Laplace evaluation path.
+ *
+ * One handle owns one HIP stream and the device-resident state of one
+ * particle system: bodies in tree order, the cell tree, multipole and local
+ * expansions and the P2P/M2L interaction lists.  Every entry point returns
+ * an int status (0 = ok, negative = error; exafmm_hip_last_error() gives the
+ * text).  A handle is not thread-safe; distinct handles are independent.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository root).  The C++ classes in
+ * exafmm-dev_amd/host/ (BoundBox, BuildTree, UpDownPass, Traversal, kernel::)
+ * forward to these calls so that an examples/serial.cxx-shaped driver runs on
+ * the GPU unchanged.
+ */
+#ifndef EXAFMM_HIP_H
+#define EXAFMM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXAFMM_HIP_OK 0
+#define EXAFMM_HIP_EINVAL (-1)   /* invalid argument or call order          */
+#define EXAFMM_HIP_ENOMEM (-2)   /* device allocation failed                */
+#define EXAFMM_HIP_EDEVICE (-3)  /* HIP runtime error                       */
+#define EXAFMM_HIP_EUNSUP (-4)   /* configuration not supported             */
+
+typedef struct exafmm_hip_handle exafmm_hip_handle;
+
+/* Run-time configuration; field names follow include/args.h:98-100. */
+typedef struct {
+  int P;          /* expansion order, one of 4, 6, 8, 10, 12 (types.h:38)      */
+  double theta;   /* MAC parameter (args.h: -t)                                */
+  int ncrit;      /* max bodies per leaf (args.h: -c)                          */
+  int nspawn;     /* split-both threshold of the traversal (args.h: -s)        */
+  int useRmax;    /* args.h: -x                                                */
+  int useRopt;    /* args.h: -o                                                */
+  int mutual;     /* args.h: -m; accepted, evaluated as the equivalent
+                     non-mutual interaction set (target-owned writes)          */
+  int images;     /* periodic image sublevels (args.h: -i)                     */
+  double cycle;   /* periodic cycle (examples/serial.cxx:12)                   */
+  int builder;    /* 0: uniform Morton tree (include/build_tree_omp.h),
+                     1: adaptive octree (include/build_tree_tbb.h)             */
+  int mass;       /* 1: MASS kernel variants (LaplaceSphericalCPU.cxx:157,200) */
+  int precision;  /* 0: FP64 throughout; 1: mixed (FP32 P2P with Kahan)       */
+  int device;     /* HIP device ordinal                                        */
+} exafmm_hip_config;
+
+/* Phase timings (ms, HIP events on the handle's stream) and work counts. */
+typedef struct {
+  double t_bounds, t_build, t_upward, t_lists, t_p2p, t_m2l, t_downward, t_total;
+  int64_t num_p2p_pairs;   /* cell pairs in the P2P list                      */
+  int64_t num_m2l_pairs;   /* cell pairs in the M2L list                      */
+  int64_t num_p2p_bodies;  /* target-source body pairs evaluated by P2P       */
+  int num_cells;
+  int num_levels;
+} exafmm_hip_stats;
+
+/* Create / destroy.  Replaces the constructors BoundBox(nspawn),
+ * BuildTree(ncrit,nspawn), UpDownPass(theta,useRmax,useRopt),
+ * Traversal(nspawn,images,eps2) of examples/serial.cxx:15-22. */
+int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out);
+int exafmm_hip_destroy(exafmm_hip_handle *h);
+const char *exafmm_hip_last_error(const exafmm_hip_handle *h);
+
+/* Load bodies (n x (x, y, z, q), caller order) from host memory, or from a
+ * device pointer on the handle's device.  Replaces filling Bodies
+ * (include/types.h:65-76; include/dataset.h:176-201). */
+int exafmm_hip_set_bodies(exafmm_hip_handle *h, int n, const double *xyzq);
+int exafmm_hip_set_bodies_device(exafmm_hip_handle *h, int n, const double *d_xyzq);
+
+/* BoundBox::getBounds (include/bound_box.h:76-88). */
+int exafmm_hip_get_bounds(exafmm_hip_handle *h, double *xmin3, double *xmax3);
+
+/* BuildTree::buildTree (include/build_tree_omp.h:216-262 or
+ * include/build_tree_tbb.h:365-374, chosen by cfg.builder).  Bodies are
+ * permuted to tree order on the device.  *ncells receives the cell count. */
+int exafmm_hip_build_tree(exafmm_hip_handle *h, const double *xmin3, const double *xmax3, int *ncells);
+
+/* UpDownPass::upwardPass (include/up_down_pass.h:119-132): P2M, M2M,
+ * setRmax, R /= theta, SetRopt. */
+int exafmm_hip_upward_pass(exafmm_hip_handle *h);
+
+/* Traversal::dualTreeTraversal (include/traversal.h:240-264): builds the
+ * P2P and M2L interaction lists on the device and evaluates them (P2P into
+ * the body results, M2L into the local expansions). */
+int exafmm_hip_traverse(exafmm_hip_handle *h);
+
+/* UpDownPass::downwardPass (include/up_down_pass.h:135-148): L2L, L2P. */
+int exafmm_hip_downward_pass(exafmm_hip_handle *h);
+
+/* All of the above on device-resident bodies, one call (the sequence of
+ * examples/serial.cxx:142-156). */
+int exafmm_hip_evaluate(exafmm_hip_handle *h);
+
+/* Readback.  Cells: ints5 = (IPARENT, ICHILD, NCHILD, IBODY, NBODY) per cell
+ * in the reference's Cell order (include/types.h:79-92); X = 3 per cell;
+ * R after the upward pass; M, L = 2*NTERM doubles per cell.  Any pointer may
+ * be NULL. */
+int exafmm_hip_get_cells(exafmm_hip_handle *h, int *ints5, uint64_t *icell, double *X, double *R, double *M,
+                         double *L);
+/* Bodies in tree order: xyzq, original index (Body::IBODY) and results
+ * (pot, dphi/dx, dphi/dy, dphi/dz), normalised as after downwardPass. */
+int exafmm_hip_get_bodies(exafmm_hip_handle *h, double *xyzq, int *ibody, double *trg);
+/* Results scattered back to the caller's body order (n x 4). */
+int exafmm_hip_get_results(exafmm_hip_handle *h, double *trg);
+/* Device pointer of the tree-order result array (n x 4 doubles). */
+int exafmm_hip_results_device(exafmm_hip_handle *h, const double **d_trg);
+
+/* Interaction lists as (target cell, source cell) pairs, grouped by target
+ * (CSR order).  Counts first, then arrays of that length. */
+int exafmm_hip_get_list_counts(exafmm_hip_handle *h, int64_t *np2p, int64_t *nm2l);
+int exafmm_hip_get_lists(exafmm_hip_handle *h, int *p2p_i, int *p2p_j, int *m2l_i, int *m2l_j);
+
+/* FP64 direct summation on the device onto nt targets given by their
+ * original body indices (Traversal::direct + normalize,
+ * include/traversal.h:266-337).  trg = nt x 4. */
+int exafmm_hip_direct(exafmm_hip_handle *h, int nt, const int *target_ibody, double *trg);
+
+int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
+
+/* Wait for all work queued on the handle's stream. */
+int exafmm_hip_synchronize(exafmm_hip_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
