@@ -127,13 +127,19 @@ static int guarded(exafmm_hip_handle *H, F &&f) {
 
 extern "C" {
 
+static int check_config(const exafmm_hip_config *cfg) {
+  if (cfg->P != 4 && cfg->P != 6 && cfg->P != 8 && cfg->P != 10 && cfg->P != 12) return EXAFMM_HIP_EUNSUP;
+  if (!(cfg->theta > 0 && cfg->theta < 1) || cfg->ncrit <= 0 || cfg->images < 0 || cfg->images > 8)
+    return EXAFMM_HIP_EINVAL;
+  if (cfg->builder != 0 && cfg->builder != 1) return EXAFMM_HIP_EINVAL;
+  if (cfg->precision != 0 && cfg->precision != 1) return EXAFMM_HIP_EINVAL;
+  return EXAFMM_HIP_OK;
+}
+
 int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   if (!cfg || !out) return EXAFMM_HIP_EINVAL;
   *out = nullptr;
-  if (cfg->P != 4 && cfg->P != 6 && cfg->P != 8 && cfg->P != 10 && cfg->P != 12) return EXAFMM_HIP_EUNSUP;
-  if (cfg->theta <= 0 || cfg->theta >= 1 || cfg->ncrit <= 0 || cfg->images < 0 || cfg->images > 8)
-    return EXAFMM_HIP_EINVAL;
-  if (cfg->builder != 0 && cfg->builder != 1) return EXAFMM_HIP_EINVAL;
+  if (int rc = check_config(cfg)) return rc;
   exafmm_hip_handle *H = new exafmm_hip_handle();
   H->h.cfg = *cfg;
   if (H->h.cfg.nspawn <= 0) H->h.cfg.nspawn = 1000;
@@ -159,6 +165,18 @@ int exafmm_hip_destroy(exafmm_hip_handle *H) {
 }
 
 const char *exafmm_hip_last_error(const exafmm_hip_handle *H) { return H ? H->h.err.c_str() : "null handle"; }
+
+int exafmm_hip_set_config(exafmm_hip_handle *H, const exafmm_hip_config *cfg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(cfg != nullptr, "set_config: null config");
+    ensure(cfg->P == h.cfg.P && cfg->device == h.cfg.device, "set_config: P and device are fixed at creation");
+    if (int rc = check_config(cfg)) throw Error(rc, "set_config: invalid configuration");
+    const bool tree_changed = cfg->ncrit != h.cfg.ncrit || cfg->builder != h.cfg.builder;
+    h.cfg = *cfg;
+    if (h.cfg.nspawn <= 0) h.cfg.nspawn = 1000;
+    if (tree_changed) h.have_tree = h.have_up = h.have_trav = false;
+  });
+}
 
 int exafmm_hip_set_bodies(exafmm_hip_handle *H, int n, const double *xyzq) {
   return guarded(H, [&](Handle &h) {
@@ -337,6 +355,14 @@ int exafmm_hip_direct(exafmm_hip_handle *H, int nt, const int *target_ibody, dou
   return guarded(H, [&](Handle &h) {
     ensure(nt >= 0 && (nt == 0 || (target_ibody && trg)), "direct: bad arguments");
     run_direct(h, nt, target_ibody, trg);
+  });
+}
+
+int exafmm_hip_direct_arrays(exafmm_hip_handle *H, int nt, const double *t_xyzq, int ns, const double *s_xyzq,
+                             double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(nt >= 0 && ns >= 0 && (nt == 0 || (t_xyzq && trg)) && (ns == 0 || s_xyzq), "direct_arrays: bad arguments");
+    run_direct_arrays(h, nt, t_xyzq, ns, s_xyzq, trg);
   });
 }
 

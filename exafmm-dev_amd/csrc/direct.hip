@@ -10,10 +10,10 @@
 namespace exafmm {
 
 __global__ void __launch_bounds__(256)
-    k_direct(const int *__restrict__ tgt, const double4 *__restrict__ src, int ns, int prange, double cycle,
-             double4 *__restrict__ out) {
+    k_direct(const int *__restrict__ tgt, const double4 *__restrict__ tgt_xyzq, const double4 *__restrict__ src,
+             int ns, int prange, double cycle, double4 *__restrict__ out) {
   __shared__ double4 red[256];
-  const double4 xi = src[tgt[blockIdx.x]];
+  const double4 xi = tgt ? src[tgt[blockIdx.x]] : tgt_xyzq[blockIdx.x];
   double pot = 0, ax = 0, ay = 0, az = 0;
   for (int ix = -prange; ix <= prange; ix++)
     for (int iy = -prange; iy <= prange; iy++)
@@ -64,11 +64,31 @@ void run_direct(Handle &h, int nt, const int *target_ibody, double *trg) {
   t.resize(nt);
   o.resize(nt);
   EXA_HIP(hipMemcpyAsync(t.p, target_ibody, nt * sizeof(int), hipMemcpyHostToDevice, h.stream));
-  hipLaunchKernelGGL(k_direct, dim3(nt), dim3(256), 0, h.stream, t.p, h.in4.p, h.n, prange, h.cfg.cycle, o.p);
+  hipLaunchKernelGGL(k_direct, dim3(nt), dim3(256), 0, h.stream, t.p, nullptr, h.in4.p, h.n, prange, h.cfg.cycle,
+                     o.p);
   EXA_HIP(hipGetLastError());
   EXA_HIP(hipMemcpyAsync(trg, o.p, nt * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
   t.release();
+  o.release();
+}
+
+void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg) {
+  if (nt <= 0) return;
+  int prange = 0;
+  for (int i = 0; i < h.cfg.images; i++) prange += int(std::pow(3., i));
+  DBuf<double4> t, s, o;
+  t.resize(nt);
+  s.resize(ns > 0 ? ns : 1);
+  o.resize(nt);
+  EXA_HIP(hipMemcpyAsync(t.p, t_xyzq, nt * sizeof(double4), hipMemcpyHostToDevice, h.stream));
+  if (ns > 0) EXA_HIP(hipMemcpyAsync(s.p, s_xyzq, ns * sizeof(double4), hipMemcpyHostToDevice, h.stream));
+  hipLaunchKernelGGL(k_direct, dim3(nt), dim3(256), 0, h.stream, nullptr, t.p, s.p, ns, prange, h.cfg.cycle, o.p);
+  EXA_HIP(hipGetLastError());
+  EXA_HIP(hipMemcpyAsync(trg, o.p, nt * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  t.release();
+  s.release();
   o.release();
 }
 

@@ -127,6 +127,7 @@ void run_p2p(Handle &h);
 void run_m2l(Handle &h);
 void run_downward(Handle &h);
 void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
+void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg);
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
 void run_periodic(Handle &h);

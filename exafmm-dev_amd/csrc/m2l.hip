@@ -9,14 +9,21 @@
 //   s = (-1)^j (-1)^(k<m ? k : m)  (m >= 0).
 // That is 8 P (P+1)^2 (P+2) / 12 flops per pair (9,680 at P = 10).
 //
+// Folding the signs into the tables, M'_n^m = (-1)^m Mhat_n^m (m >= 0),
+// M'_n^m = Mhat_n^m (m < 0), S'_a^b = (-1)^b S_a^b (b > 0), S'_a^b = S_a^b
+// (b <= 0), leaves a plain correlation in k:
+//   L_j^k += (-1)^j sum_n sum_m M'_n^m S'_{j+n}^{m-k}.
+//
 // Work split: the (j, k, n) triples -- each a run of 2n+1 complex MACs --
 // are packed onto the 64 lanes by longest-processing-time-first on the
 // host, so every lane does ~1/64 of the pair's MACs (the per-output split
 // would leave lane (0,0) with 100 MACs and lane (9,k) with 1 at P = 10).
-// Per source the wavefront writes the P^2-entry tables Mhat and S into LDS,
-// each lane runs its fixed item list (descriptors held in registers) and
-// adds each finished triple into its own LDS slot; after the last source the
-// slots of each output are summed in n order.
+// Per source the wavefront writes the P^2-entry tables M' and S' into LDS
+// (one packed coefficient per lane, division-free recurrences), each lane
+// runs its fixed item list (descriptors held in registers) and adds each
+// finished triple into its own LDS slot; after the last source the slots of
+// each output are summed in n order.  The next source's multipole and
+// centre are loaded while the current pair is contracted.
 #include <algorithm>
 #include <vector>
 
@@ -42,61 +49,80 @@ struct M2LShape {
   static constexpr int IPL = (TOTAL + 63) / 64 > MAXTRIPLE ? (TOTAL + 63) / 64 + 3 : MAXTRIPLE + 1;
 };
 
-// Descriptor bits: [0,8) Mhat index, [8,16) S index, [16] negate,
-// [17] end of triple, [18,27) triple id.  A padding item points both
-// indices at the zero entry NF and never ends a triple.
-static std::vector<uint32_t> build_schedule(int P, int ipl) {
+// Lane schedule.  Each lane owns a list of (j, k, n) triples (LPT packing by
+// 2n+1 MACs); its items run m = -n..n of each triple in order.  Descriptor:
+//   bits [0,12)  byte offset of M'_n^m in the M' table (16 B entries)
+//   bits [12,24) byte offset of S'_{j+n}^{m-k} in the S' table
+//   bit 24       last item of a triple: add the partial into the lane's next
+//                accumulator slot
+// Padding items point at the zero entry NF of both tables.  Slots are laid
+// out lane-major (lane l owns slots [slot0[l], slot0[l] + #triples(l))), and
+// `outslot` lists, for every output (j, k), its slots in n order.
+struct M2LSchedule {
+  std::vector<uint32_t> desc;     // IPL x 64, item-major
+  std::vector<int> slot0;         // 64
+  std::vector<int> outslot;       // NT x P (-1 padded)
+  int nslots = 0;
+};
+
+static M2LSchedule build_schedule(int P, int ipl) {
   struct Triple {
-    int j, k, n, id, work;
+    int j, k, n, work;
   };
   std::vector<Triple> tr;
-  int id = 0;
   for (int j = 0; j < P; j++)
     for (int k = 0; k <= j; k++)
-      for (int n = 0; n < P - j; n++) tr.push_back({j, k, n, id++, 2 * n + 1});
-  std::vector<Triple> order = tr;
-  std::stable_sort(order.begin(), order.end(), [](const Triple &a, const Triple &b) { return a.work > b.work; });
+      for (int n = 0; n < P - j; n++) tr.push_back({j, k, n, 2 * n + 1});
+  std::vector<int> order(tr.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tr[a].work > tr[b].work; });
   std::vector<std::vector<int>> lanes(64);
   std::vector<int> load(64, 0);
-  for (const Triple &t : order) {
+  for (int t : order) {
     int best = 0;
     for (int l = 1; l < 64; l++)
       if (load[l] < load[best]) best = l;
-    lanes[best].push_back(t.id);
-    load[best] += t.work;
+    lanes[best].push_back(t);
+    load[best] += tr[t].work;
   }
-  const int NF = P * P;
-  std::vector<uint32_t> desc(64 * (size_t)ipl, (uint32_t)NF | ((uint32_t)NF << 8));
+  const int NF = P * P, NT = P * (P + 1) / 2;
+  M2LSchedule s;
+  const uint32_t pad = (uint32_t)(NF * 16) | ((uint32_t)(NF * 16) << 12);
+  s.desc.assign(64 * (size_t)ipl, pad);
+  s.slot0.assign(64, 0);
+  s.outslot.assign((size_t)NT * P, -1);
+  int slot = 0;
   for (int l = 0; l < 64; l++) {
     if (load[l] > ipl) throw Error(EXAFMM_HIP_EUNSUP, "M2L schedule exceeds the per-lane item budget");
+    s.slot0[l] = slot;
     int it = 0;
-    for (int tid : lanes[l]) {
-      const Triple &t = tr[tid];
-      for (int m = -t.n; m <= t.n; m++) {
-        const int a = t.j + t.n, b = m - t.k;
-        const uint32_t mi = t.n * t.n + t.n + m;
-        const uint32_t yi = a * a + a + b;
-        int sgn = (t.j & 1) ? -1 : 1;
-        if (m >= 0) sgn *= (((t.k < m) ? t.k : m) & 1) ? -1 : 1;
-        uint32_t d = mi | (yi << 8) | ((sgn < 0 ? 1u : 0u) << 16) | ((uint32_t)t.id << 18);
-        if (m == t.n) d |= 1u << 17;
-        desc[(size_t)it * 64 + l] = d;
+    for (int t : lanes[l]) {
+      const Triple &q = tr[t];
+      for (int m = -q.n; m <= q.n; m++) {
+        const int a = q.j + q.n, b = m - q.k;
+        uint32_t d = (uint32_t)((q.n * q.n + q.n + m) * 16) | ((uint32_t)((a * a + a + b) * 16) << 12);
+        if (m == q.n) d |= 1u << 24;
+        s.desc[(size_t)it * 64 + l] = d;
         it++;
       }
+      s.outslot[(size_t)(q.j * (q.j + 1) / 2 + q.k) * P + q.n] = slot++;
     }
   }
-  return desc;
+  s.nslots = slot;
+  return s;
 }
 
 template <int P>
 __global__ void __launch_bounds__(64)
     k_m2l_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
               const unsigned char *__restrict__ img, const double4 *__restrict__ cX, const cplx *__restrict__ M,
-              cplx *__restrict__ L, const uint32_t *__restrict__ sched, double cycle, int mass) {
+              cplx *__restrict__ L, const uint32_t *__restrict__ sched, const int *__restrict__ slot0,
+              const int *__restrict__ outslot, double cycle, int mass) {
   using S = M2LShape<P>;
   constexpr int NT = Expn<P>::NT;
   constexpr int NF = S::NF;
   constexpr int IPL = S::IPL;
+  constexpr int NO = (NT + 63) / 64;
   __shared__ cplx sM[2][NF + 1];
   __shared__ cplx sS[2][NF + 1];
   __shared__ cplx seg[S::NTRIPLE];
@@ -105,52 +131,107 @@ __global__ void __launch_bounds__(64)
   uint32_t desc[IPL];
 #pragma unroll
   for (int it = 0; it < IPL; it++) desc[it] = sched[it * 64 + lane];
+  const int myslot0 = slot0[lane];
   for (int t = lane; t < S::NTRIPLE; t += 64) seg[t] = {0, 0};
   if (lane == 0) {
     sM[0][NF] = sM[1][NF] = {0, 0};
     sS[0][NF] = sS[1][NF] = {0, 0};
   }
+  // Packed coefficients (n, m >= 0) this lane translates: t = lane + 64 o.
+  int tn[NO], tm[NO];
+#pragma unroll
+  for (int o = 0; o < NO; o++) {
+    const int t = lane + 64 * o < NT ? lane + 64 * o : 0;
+    int n = 0;
+    while (tri(n + 1) <= t) n++;
+    tn[o] = n;
+    tm[o] = t - tri(n);
+  }
   const double4 Xi = cX[c];
   const double mi0 = mass ? M[(size_t)c * NT].re : 1.0;
   const int e0 = off[c], e1 = off[c + 1];
+  // Prefetch of the first source.
+  int s_next = e0 < e1 ? src[e0] : 0;
+  int im_next = e0 < e1 ? img[e0] : 13;
+  cplx m_next[NO];
+#pragma unroll
+  for (int o = 0; o < NO; o++) m_next[o] = (e0 < e1 && lane + 64 * o < NT) ? M[(size_t)s_next * NT + lane + 64 * o] : cplx{0, 0};
+  double4 X_next = cX[s_next];
+  double mj0_next = mass ? M[(size_t)s_next * NT].re : 1.0;
   for (int e = e0; e < e1; e++) {
     const int buf = (e - e0) & 1;
-    const int s = src[e];
-    const double4 Xj = cX[s];
-    const int im = img[e];
-    const double px = (im / 9 - 1) * cycle, py = ((im / 3) % 3 - 1) * cycle, pz = (im % 3 - 1) * cycle;
-    const Geom g = make_geom(Xi.x - Xj.x - px, Xi.y - Xj.y - py, Xi.z - Xj.z - pz);
-    for (int f = lane; f < NF; f += 64) {
-      int a = 0;
-      while ((a + 1) * (a + 1) <= f) a++;
-      const int b = f - a * a - a;
-      const int ab = b >= 0 ? b : -b;
-      double re, imv;
-      singular_nm(a, ab, g, re, imv);
-      sS[buf][f] = {re, b >= 0 ? imv : -imv};
-      cplx mv = M[(size_t)s * NT + tri(a) + ab];
-      if (b < 0) mv.im = -mv.im;
-      if (mass && f == 0) mv = {1.0, 0.0};
-      sM[buf][f] = mv;
+    const double4 Xj = X_next;
+    const int im = im_next;
+    cplx mcur[NO];
+#pragma unroll
+    for (int o = 0; o < NO; o++) mcur[o] = m_next[o];
+    const double mj0 = mj0_next;
+    if (e + 1 < e1) {  // issue the next pair's loads now
+      s_next = src[e + 1];
+      im_next = img[e + 1];
+#pragma unroll
+      for (int o = 0; o < NO; o++) m_next[o] = lane + 64 * o < NT ? M[(size_t)s_next * NT + lane + 64 * o] : cplx{0, 0};
+      X_next = cX[s_next];
+      if (mass) mj0_next = M[(size_t)s_next * NT].re;
     }
-    const double scale = mass ? mi0 * M[(size_t)s * NT].re : 1.0;
+    const double px = (im / 9 - 1) * cycle, py = ((im / 3) % 3 - 1) * cycle, pz = (im % 3 - 1) * cycle;
+    const double dx = Xi.x - Xj.x - px, dy = Xi.y - Xj.y - py, dz = Xi.z - Xj.z - pz;
+    // Geometry without divisions: u = 1/r, w = 1/rho_xy.
+    const double rxy2 = dx * dx + dy * dy;
+    const double u = rsqrt(rxy2 + dz * dz);
+    const double w = rxy2 > 0 ? rsqrt(rxy2) : 0.0;
+    const double ct = dz * u, su = rxy2 * w * u * u;  // cos(alpha), sin(alpha) / r
+    const double cp = rxy2 > 0 ? dx * w : 1.0, sp = dy * w;
+    const double cu = ct * u, u2 = u * u;
+#pragma unroll
+    for (int o = 0; o < NO; o++) {
+      if (lane + 64 * o >= NT) continue;
+      const int a = tn[o], b = tm[o];
+      // S_a^b = (-1)^a (a-b)! u^(a+1) P_a^b(ct) e^{ib phi} = G_a e^{ib phi}, with
+      // G_b = (2b-1)!! (st u)^b u and the division-free upward recurrence
+      // G_a = -(2a-1) ct u G_{a-1} - (a+b-1)(a-b-1) u^2 G_{a-2}.
+      double g = u, gm1 = 0, ec = 1, es = 0;
+      for (int i = 1; i <= b; i++) {
+        g *= su * (2 * i - 1);
+        const double t = ec * cp - es * sp;
+        es = ec * sp + es * cp;
+        ec = t;
+      }
+      for (int t = b + 1; t <= a; t++) {
+        const double gn = -(2 * t - 1) * cu * g - (double)((t + b - 1) * (t - b - 1)) * u2 * gm1;
+        gm1 = g;
+        g = gn;
+      }
+      const double sr = g * ec, si = g * es;
+      // S'_a^b = (-1)^b S_a^b for b > 0, S'_a^{-b} = conj(S_a^b).
+      const double sg = (b & 1) ? -1.0 : 1.0;
+      sS[buf][a * a + a + b] = {sg * sr, sg * si};
+      if (b > 0) sS[buf][a * a + a - b] = {sr, -si};
+      // M'_n^m = (-1)^m M_n^m (m >= 0), M'_n^{-m} = conj(M_n^m).
+      cplx mv = mcur[o];
+      if (mass && a == 0) mv = {1.0, 0.0};
+      sM[buf][a * a + a + b] = {sg * mv.re, sg * mv.im};
+      if (b > 0) sM[buf][a * a + a - b] = {mv.re, -mv.im};
+    }
+    const double scale = mass ? mi0 * mj0 : 1.0;
     __syncthreads();
+    const char *mb = reinterpret_cast<const char *>(sM[buf]);
+    const char *sb = reinterpret_cast<const char *>(sS[buf]);
     double ar = 0, ai = 0;
+    int slot = myslot0;
 #pragma unroll
     for (int it = 0; it < IPL; it++) {
       const uint32_t d = desc[it];
-      const cplx mv = sM[buf][d & 255];
-      const cplx sv = sS[buf][(d >> 8) & 255];
-      const double sg = (d >> 16) & 1 ? -1.0 : 1.0;
-      const double mr = sg * mv.re, mim = sg * mv.im;
-      ar += mr * sv.re - mim * sv.im;
-      ai += mr * sv.im + mim * sv.re;
-      if ((d >> 17) & 1) {
-        const int tid = d >> 18;
-        cplx t = seg[tid];
+      const cplx mv = *reinterpret_cast<const cplx *>(mb + (d & 0xFFF));
+      const cplx sv = *reinterpret_cast<const cplx *>(sb + ((d >> 12) & 0xFFF));
+      ar += mv.re * sv.re - mv.im * sv.im;
+      ai += mv.re * sv.im + mv.im * sv.re;
+      if (d & (1u << 24)) {
+        cplx t = seg[slot];
         t.re += scale * ar;
         t.im += scale * ai;
-        seg[tid] = t;
+        seg[slot] = t;
+        slot++;
         ar = ai = 0;
       }
     }
@@ -159,18 +240,16 @@ __global__ void __launch_bounds__(64)
   for (int t = lane; t < NT; t += 64) {
     int j = 0;
     while (tri(j + 1) <= t) j++;
-    const int k = t - tri(j);
-    int base = 0;
-    for (int jj = 0; jj < j; jj++) base += (jj + 1) * (P - jj);
-    base += k * (P - j);
     double lr = 0, li = 0;
     for (int n = 0; n < P - j; n++) {
-      lr += seg[base + n].re;
-      li += seg[base + n].im;
+      const cplx v = seg[outslot[t * P + n]];
+      lr += v.re;
+      li += v.im;
     }
+    const double sgn = (j & 1) ? -1.0 : 1.0;
     cplx o = L[(size_t)c * NT + t];
-    o.re += lr;
-    o.im += li;
+    o.re += sgn * lr;
+    o.im += sgn * li;
     L[(size_t)c * NT + t] = o;
   }
 }
@@ -179,15 +258,21 @@ template <int P>
 static void m2l_impl(Handle &h) {
   using S = M2LShape<P>;
   if (h.m2l_sched_P != P) {
-    const std::vector<uint32_t> host = build_schedule(P, S::IPL);
-    h.m2l_sched.resize(host.size());
-    EXA_HIP(hipMemcpyAsync(h.m2l_sched.p, host.data(), host.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+    const M2LSchedule sc = build_schedule(P, S::IPL);
+    std::vector<uint32_t> blob(sc.desc);
+    for (int v : sc.slot0) blob.push_back((uint32_t)v);
+    for (int v : sc.outslot) blob.push_back((uint32_t)v);
+    h.m2l_sched.resize(blob.size());
+    EXA_HIP(hipMemcpyAsync(h.m2l_sched.p, blob.data(), blob.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                            h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
     h.m2l_sched_P = P;
   }
+  const uint32_t *desc = h.m2l_sched.p;
+  const int *slot0 = reinterpret_cast<const int *>(desc + 64 * S::IPL);
+  const int *outslot = slot0 + 64;
   hipLaunchKernelGGL(k_m2l_f64<P>, dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p, h.m2l_src.p,
-                     h.m2l_img.p, h.cX.p, h.M.p, h.L.p, h.m2l_sched.p, h.cfg.cycle, h.cfg.mass);
+                     h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
   EXA_HIP(hipGetLastError());
 }
 

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libexafmm_hip.so")
 
 # Every symbol declared in include/exafmm_hip.h.
 EXPORTS = (
-    "exafmm_hip_create", "exafmm_hip_destroy", "exafmm_hip_last_error", "exafmm_hip_set_bodies",
+    "exafmm_hip_create", "exafmm_hip_destroy", "exafmm_hip_set_config", "exafmm_hip_last_error",
+    "exafmm_hip_set_bodies", "exafmm_hip_direct_arrays",
     "exafmm_hip_set_bodies_device", "exafmm_hip_get_bounds", "exafmm_hip_build_tree", "exafmm_hip_upward_pass",
     "exafmm_hip_traverse", "exafmm_hip_downward_pass", "exafmm_hip_evaluate", "exafmm_hip_get_cells",
     "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
@@ -87,6 +88,8 @@ def load():
     L.exafmm_hip_get_list_counts.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64)]
     L.exafmm_hip_get_lists.argtypes = [vp, vp, vp, vp, vp]
     L.exafmm_hip_direct.argtypes = [vp, ip, vp, vp]
+    L.exafmm_hip_direct_arrays.argtypes = [vp, ip, vp, ip, vp, vp]
+    L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     for f in EXPORTS:
         if f != "exafmm_hip_last_error":
@@ -218,6 +221,14 @@ class FMM:
         t = np.ascontiguousarray(target_ibody, np.int32)
         out = np.zeros((t.shape[0], 4))
         self._check(self.lib.exafmm_hip_direct(self.h, t.shape[0], _ptr(t), _ptr(out)), "direct")
+        return out
+
+    def direct_arrays(self, targets, sources):
+        t = np.ascontiguousarray(targets, np.float64)
+        s = np.ascontiguousarray(sources, np.float64)
+        out = np.zeros((t.shape[0], 4))
+        self._check(self.lib.exafmm_hip_direct_arrays(self.h, t.shape[0], _ptr(t), s.shape[0], _ptr(s), _ptr(out)),
+                    "direct_arrays")
         return out
 
     def stats(self):

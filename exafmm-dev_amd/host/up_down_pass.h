@@ -1,0 +1,68 @@
+// UpDownPass (include/up_down_pass.h:7-170): upward and downward passes on
+// the device.  upwardPass refreshes Cell::R; downwardPass adds the results
+// into Body::TRG of the tree-ordered bodies (the reference accumulates into
+// TRG and normalises by SRC in L2P).
+#ifndef up_down_pass_h
+#define up_down_pass_h
+#include <vector>
+
+#include "hip_context.h"
+#include "logger.h"
+#include "thread.h"
+#include "types.h"
+
+class UpDownPass {
+  const real_t theta;
+  const bool useRmax;
+  const bool useRopt;
+
+ public:
+  UpDownPass(real_t _theta, bool _useRmax, bool _useRopt) : theta(_theta), useRmax(_useRmax), useRopt(_useRopt) {}
+
+  void upwardPass(Cells &cells) {
+    logger::startTimer("Upward pass");
+    if (!cells.empty()) {
+      exafmm_host::Context &ctx = exafmm_host::context();
+      ctx.cfg.theta = theta;
+      ctx.cfg.useRmax = useRmax;
+      ctx.cfg.useRopt = useRopt;
+      ctx.apply();
+      ctx.check(exafmm_hip_upward_pass(ctx.h), "exafmm_hip_upward_pass");
+      std::vector<double> R(cells.size());
+      ctx.check(exafmm_hip_get_cells(ctx.h, nullptr, nullptr, nullptr, R.data(), nullptr, nullptr),
+                "exafmm_hip_get_cells");
+      for (size_t c = 0; c < cells.size(); c++) cells[c].R = R[c];
+    }
+    logger::stopTimer("Upward pass");
+  }
+
+  void downwardPass(Cells &cells) {
+    logger::startTimer("Downward pass");
+    if (!cells.empty()) {
+      exafmm_host::Context &ctx = exafmm_host::context();
+      ctx.check(exafmm_hip_downward_pass(ctx.h), "exafmm_hip_downward_pass");
+      const int n = cells.front().NBODY;
+      std::vector<double> trg(4 * (size_t)n);
+      ctx.check(exafmm_hip_get_bodies(ctx.h, nullptr, nullptr, trg.data()), "exafmm_hip_get_bodies");
+      B_iter B0 = cells.front().BODY;
+      for (int i = 0; i < n; i++)
+        for (int d = 0; d < 4; d++) B0[i].TRG[d] += trg[4 * (size_t)i + d];
+    }
+    logger::stopTimer("Downward pass");
+  }
+
+  // Copy the device expansions into Cell::M and Cell::L (host inspection).
+  void fetchExpansions(Cells &cells) {
+    exafmm_host::Context &ctx = exafmm_host::context();
+    std::vector<complex_t> M(cells.size() * NTERM), L(cells.size() * NTERM);
+    ctx.check(exafmm_hip_get_cells(ctx.h, nullptr, nullptr, nullptr, nullptr, reinterpret_cast<double *>(M.data()),
+                                   reinterpret_cast<double *>(L.data())),
+              "exafmm_hip_get_cells");
+    for (size_t c = 0; c < cells.size(); c++)
+      for (int t = 0; t < NTERM; t++) {
+        cells[c].M[t] = M[c * NTERM + t];
+        cells[c].L[t] = L[c * NTERM + t];
+      }
+  }
+};
+#endif

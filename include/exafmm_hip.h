@@ -62,6 +62,11 @@ typedef struct {
  * Traversal(nspawn,images,eps2) of examples/serial.cxx:15-22. */
 int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out);
 int exafmm_hip_destroy(exafmm_hip_handle *h);
+/* Change the run-time parameters between passes (theta, ncrit, nspawn,
+ * useRmax, useRopt, mutual, images, cycle, builder, mass, precision); P and
+ * device are fixed at creation.  The reference reads them from the pass
+ * objects' constructors (examples/serial.cxx:15-22). */
+int exafmm_hip_set_config(exafmm_hip_handle *h, const exafmm_hip_config *cfg);
 const char *exafmm_hip_last_error(const exafmm_hip_handle *h);
 
 /* Load bodies (n x (x, y, z, q), caller order) from host memory, or from a
@@ -117,6 +122,11 @@ int exafmm_hip_get_lists(exafmm_hip_handle *h, int *p2p_i, int *p2p_j, int *m2l_
  * original body indices (Traversal::direct + normalize,
  * include/traversal.h:266-337).  trg = nt x 4. */
 int exafmm_hip_direct(exafmm_hip_handle *h, int nt, const int *target_ibody, double *trg);
+/* The same on explicit arrays: nt targets and ns sources (x, y, z, q each),
+ * periodic images and cycle taken from the handle.  trg = nt x 4, normalised
+ * by the target charge. */
+int exafmm_hip_direct_arrays(exafmm_hip_handle *h, int nt, const double *t_xyzq, int ns, const double *s_xyzq,
+                             double *trg);
 
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 
