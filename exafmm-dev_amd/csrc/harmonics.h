@@ -51,12 +51,27 @@ __device__ inline Geom make_geom(double dx, double dy, double dz) {
   return g;
 }
 
-// 1/k! for k < 32.
-__device__ inline double inv_factorial(int k) {
-  double f = 1;
-  for (int i = 2; i <= k; i++) f /= i;
-  return f;
-}
+namespace detail {
+template <int N>
+struct Reciprocals {
+  double v[N];
+  constexpr Reciprocals() : v() {
+    for (int i = 1; i < N; i++) v[i] = 1.0 / i;
+  }
+};
+template <int N>
+struct InvFactorials {
+  double v[N];
+  constexpr InvFactorials() : v() {
+    v[0] = 1;
+    for (int i = 1; i < N; i++) v[i] = v[i - 1] / i;
+  }
+};
+}  // namespace detail
+
+// 1/k (k < 256) and 1/k! (k < 32), so the recurrences below never divide.
+__constant__ constexpr detail::Reciprocals<256> kInv{};
+__constant__ constexpr detail::InvFactorials<32> kInvFact{};
 
 __device__ inline double factorial(int k) {
   double f = 1;
@@ -70,7 +85,7 @@ __device__ inline double legendre(int n, int m, double ct, double st) {
   for (int i = 1; i <= m; i++) pmm *= -(2 * i - 1) * st;
   double p = pmm, p1 = 0;
   for (int k = m; k < n; k++) {
-    const double pn = (ct * (2 * k + 1) * p - (k + m) * p1) / (k - m + 1);
+    const double pn = (ct * (2 * k + 1) * p - (k + m) * p1) * kInv.v[k - m + 1];
     p1 = p;
     p = pn;
   }
@@ -88,16 +103,25 @@ __device__ inline void eim(int m, double cp, double sp, double &c, double &s) {
   }
 }
 
-// One regular harmonic R_n^m, m >= 0.
+// One regular harmonic R_n^m, m >= 0, without divisions:
+//   H_m = (r sin(a) / 2)^m / m!,  H_{m+1} = -r cos(a) H_m,
+//   H_k = -((2k-1) r cos(a) H_{k-1} + r^2 H_{k-2}) / (k^2 - m^2),
+// R_n^m = H_n e^{i m b}.
 __device__ inline void regular_nm(int n, int m, const Geom &g, double &re, double &im) {
-  double sc = inv_factorial(n + m);
-  for (int i = 0; i < n; i++) sc *= g.r;
-  if (n & 1) sc = -sc;
-  const double p = legendre(n, m, g.ct, g.st) * sc;
+  const double rs = 0.5 * g.r * g.st;
+  double h = kInvFact.v[m];
+  for (int i = 0; i < m; i++) h *= rs;
+  double hm1 = 0;
+  const double cr = g.ct * g.r, r2 = g.r * g.r;
+  for (int k = m + 1; k <= n; k++) {
+    const double hn = -((2 * k - 1) * cr * h + r2 * hm1) * kInv.v[k * k - m * m];
+    hm1 = h;
+    h = hn;
+  }
   double c, s;
   eim(m, g.cp, g.sp, c, s);
-  re = p * c;
-  im = p * s;
+  re = h * c;
+  im = h * s;
 }
 
 // One singular harmonic S_n^m, m >= 0.

@@ -97,6 +97,8 @@ struct Handle {
   DBuf<unsigned char> p2p_img, m2l_img;
   DBuf<int> p2p_tgt, m2l_tgt;      // compact target lists
   int np2p_tgt = 0, nm2l_tgt = 0;
+  DBuf<int> raw_mi, raw_mj, raw_pi, raw_pj;  // emission-order lists
+  DBuf<unsigned char> raw_mimg, raw_pimg;
 
   // M2L lane schedule (a pure function of P).
   DBuf<uint32_t> m2l_sched;

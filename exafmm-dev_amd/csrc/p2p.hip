@@ -10,8 +10,9 @@
 // Lane layout: a leaf's targets are replicated over G = 64 / T lane groups
 // (T = 16, 32 or 64 targets per group) and group g takes every G-th source
 // body, so short leaves still fill the wavefront.  Source bodies are staged
-// through LDS one source leaf at a time; the G partial sums are combined in a
-// fixed order at the end (deterministic, no atomics).
+// through LDS up to 256 at a time (consecutive list entries, each shifted by
+// its periodic image); the G partial sums are combined in a fixed order at
+// the end (deterministic, no atomics).
 #include "internal.h"
 
 namespace exafmm {
@@ -22,11 +23,13 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
   pz = (img % 3 - 1) * cycle;
 }
 
+constexpr int kP2PStage = 256;  // source bodies staged in LDS per fill
+
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
               const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
               const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
-  __shared__ double4 sb[64];
+  __shared__ double4 sb[kP2PStage];
   __shared__ double4 sacc[64];
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
@@ -37,33 +40,47 @@ __global__ void __launch_bounds__(64)
   const int e0 = off[c], e1 = off[c + 1];
   for (int tbase = 0; tbase < tn; tbase += T) {
     const bool active = tbase + ti < tn;
-    double4 xi = active ? body[tb0 + tbase + ti] : make_double4(0, 0, 0, 0);
+    const double4 xi = active ? body[tb0 + tbase + ti] : make_double4(0, 0, 0, 0);
     double pot = 0, ax = 0, ay = 0, az = 0;
-    for (int e = e0; e < e1; e++) {
-      const int s = src[e];
-      double px = 0, py = 0, pz = 0;
-      image_shift(img[e], cycle, px, py, pz);
-      const double xix = xi.x - px, xiy = xi.y - py, xiz = xi.z - pz;
-      const int sb0 = cBody[s], sn = cNBody[s];
-      for (int sbase = 0; sbase < sn; sbase += 64) {
-        const int cnt = min(64, sn - sbase);
-        __syncthreads();
-        if (lane < cnt) sb[lane] = body[sb0 + sbase + lane];
-        __syncthreads();
-        for (int k = g; k < cnt; k += G) {
-          const double4 xj = sb[k];
-          const double dx = xix - xj.x, dy = xiy - xj.y, dz = xiz - xj.z;
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 != 0) {
-            const double invr = rsqrt(r2);
-            const double qinvr = xj.w * invr;
-            const double qinvr3 = qinvr * invr * invr;
-            pot += qinvr;
-            ax += dx * qinvr3;
-            ay += dy * qinvr3;
-            az += dz * qinvr3;
-          }
+    int e = e0, sbase = 0;
+    while (e < e1) {
+      // Fill: consecutive source leaves (or parts of one) up to kP2PStage
+      // bodies, each shifted by its periodic image.
+      int cnt = 0;
+      __syncthreads();
+      while (e < e1 && cnt < kP2PStage) {
+        const int s = src[e];
+        const int sb0 = cBody[s], sn = cNBody[s];
+        double px = 0, py = 0, pz = 0;
+        image_shift(img[e], cycle, px, py, pz);
+        const int take = min(sn - sbase, kP2PStage - cnt);
+        for (int k = lane; k < take; k += 64) {
+          double4 v = body[sb0 + sbase + k];
+          v.x += px;
+          v.y += py;
+          v.z += pz;
+          sb[cnt + k] = v;
         }
+        cnt += take;
+        sbase += take;
+        if (sbase == sn) {
+          e++;
+          sbase = 0;
+        }
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int k = g; k < cnt; k += G) {
+        const double4 xj = sb[k];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double invr = r2 != 0 ? rsqrt(r2) : 0.0;
+        const double qinvr = xj.w * invr;
+        const double qinvr3 = qinvr * invr * invr;
+        pot += qinvr;
+        ax += dx * qinvr3;
+        ay += dy * qinvr3;
+        az += dz * qinvr3;
       }
     }
     // Combine the G group partials in group order.

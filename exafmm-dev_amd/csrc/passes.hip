@@ -279,14 +279,14 @@ __global__ void __launch_bounds__(64)
       double rhon = rhom;
 #pragma unroll
       for (int n = m + 1; n < P; n++) {
-        rhon /= -(n + m);
+        rhon *= -1.0 / (n + m);  // compile-time reciprocals after unrolling
         accumulate(n, m, rhon * p * er, rhon * p * ei);
         const double p2 = p1;
         p1 = p;
-        p = (x * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+        p = (x * (2 * n + 1) * p1 - (n + m) * p2) * (1.0 / (n - m + 1));
         rhon *= rho;
       }
-      rhom /= -(2 * m + 2) * (2 * m + 1);
+      rhom *= -1.0 / ((2 * m + 2) * (2 * m + 1));
       pn = -pn * fact * y;
       fact += 2;
       const double t = er * g.cp - ei * g.sp;

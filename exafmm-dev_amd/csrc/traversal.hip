@@ -7,8 +7,8 @@
 // reference's rule -- M2L when |Xi - Xj - Xp|^2 > (Ri + Rj)^2, P2P when both
 // are leaves, otherwise the split of splitCell (:188-212) -- and compacts the
 // three outputs with exclusive scans, so the lists are deterministic.  The
-// non-mutual recursion visits the same set of pairs as the reference's
-// TraverseRange quadrant order; only the order differs, and the lists are
+// frontier visits the same set of pairs as the reference's TraverseRange
+// recursion (mutual or not); only the order differs, and the lists are
 // regrouped by target cell (CSR) afterwards.
 #include "internal.h"
 
@@ -26,7 +26,7 @@ __device__ inline void image_offset(int img, double cycle, double &px, double &p
 }
 
 // action: 0 M2L, 1 P2P, 2 split Ci, 3 split Cj, 4 split both.
-__device__ inline int classify(const CellView &cv, int ci, int cj, int img, double cycle, int nspawn) {
+__device__ inline int classify(const CellView &cv, int ci, int cj, int img, double cycle, int nspawn, int mutual) {
 #pragma clang fp contract(off)
   const double4 Xi = cv.X[ci], Xj = cv.X[cj];
   double px, py, pz;
@@ -44,11 +44,15 @@ __device__ inline int classify(const CellView &cv, int ci, int cj, int img, doub
   if (li && lj) return cv.nbody[cj] == 0 ? 0 : 1;
   if (lj) return 2;
   if (li) return 3;
-  if (cv.nbody[ci] + cv.nbody[cj] >= nspawn) return 4;
+  if (cv.nbody[ci] + cv.nbody[cj] >= nspawn || (mutual && ci == cj)) return 4;
   return Xi.w >= Xj.w ? 2 : 3;
 }
 
-__global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn,
+// With mutual (the reference's -m 1), a visited pair (Ci, Cj), Ci != Cj,
+// stands for both directions: M2L and P2P emit both directed entries, and a
+// cell split against itself visits only the child pairs (a, b), a <= b, as
+// TraverseRange skips the (latter, former) quadrant (traversal.h:108-120).
+__global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
                            int *__restrict__ act, int *__restrict__ c_m2l, int *__restrict__ c_p2p,
                            int *__restrict__ c_next) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,21 +62,24 @@ __global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, dou
     return;
   }
   const int4 f = fr[i];
-  const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn);
+  const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
+  const int both = (mutual && f.x != f.y) ? 2 : 1;
   act[i] = a;
-  c_m2l[i] = a == 0;
-  c_p2p[i] = a == 1;
+  c_m2l[i] = a == 0 ? both : 0;
+  c_p2p[i] = a == 1 ? both : 0;
   int nx = 0;
-  if (a == 2)
+  if (a == 2) {
     nx = cv.nchild[f.x];
-  else if (a == 3)
+  } else if (a == 3) {
     nx = cv.nchild[f.y];
-  else if (a == 4)
-    nx = cv.nchild[f.x] * cv.nchild[f.y];
+  } else if (a == 4) {
+    const int nci = cv.nchild[f.x];
+    nx = (mutual && f.x == f.y) ? nci * (nci + 1) / 2 : nci * cv.nchild[f.y];
+  }
   c_next[i] = nx;
 }
 
-__global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, const int *__restrict__ act,
+__global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
                        const int *__restrict__ o_m2l, const int *__restrict__ o_p2p, const int *__restrict__ o_next,
                        int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
                        unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
@@ -81,16 +88,27 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, const i
   if (i >= nf) return;
   const int4 f = fr[i];
   const int a = act[i];
+  const bool both = mutual && f.x != f.y;
   if (a == 0) {
     const int64_t o = base_m2l + o_m2l[i];
     m2l_i[o] = f.x;
     m2l_j[o] = f.y;
     m2l_img[o] = (unsigned char)f.z;
+    if (both) {
+      m2l_i[o + 1] = f.y;
+      m2l_j[o + 1] = f.x;
+      m2l_img[o + 1] = (unsigned char)f.z;
+    }
   } else if (a == 1) {
     const int64_t o = base_p2p + o_p2p[i];
     p2p_i[o] = f.x;
     p2p_j[o] = f.y;
     p2p_img[o] = (unsigned char)f.z;
+    if (both) {
+      p2p_i[o + 1] = f.y;
+      p2p_j[o + 1] = f.x;
+      p2p_img[o + 1] = (unsigned char)f.z;
+    }
   } else {
     int o = o_next[i];
     if (a == 2) {
@@ -102,8 +120,13 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, const i
     } else {
       const int ci0 = cv.child[f.x], nci = cv.nchild[f.x];
       const int cj0 = cv.child[f.y], ncj = cv.nchild[f.y];
-      for (int a1 = 0; a1 < nci; a1++)
-        for (int b1 = 0; b1 < ncj; b1++) next[o++] = make_int4(ci0 + a1, cj0 + b1, f.z, 0);
+      if (mutual && f.x == f.y) {
+        for (int a1 = 0; a1 < nci; a1++)
+          for (int b1 = a1; b1 < nci; b1++) next[o++] = make_int4(ci0 + a1, ci0 + b1, f.z, 0);
+      } else {
+        for (int a1 = 0; a1 < nci; a1++)
+          for (int b1 = 0; b1 < ncj; b1++) next[o++] = make_int4(ci0 + a1, cj0 + b1, f.z, 0);
+      }
     }
   }
 }
@@ -117,10 +140,28 @@ __global__ void k_init_frontier(int4 *__restrict__ fr, int images) {
   }
 }
 
-// Histogram of targets (counts are order-independent, hence deterministic).
-__global__ void k_count_targets(const int *__restrict__ tgt, int64_t n, int *__restrict__ cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[tgt[i]], 1);
+// CSR offsets from the target-sorted keys: off[c] = first index with key >= c.
+__global__ void k_csr_offsets(const int *__restrict__ keys, int64_t n, int nc, int *__restrict__ off) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nc) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  off[c] = (int)lo;
+}
+
+__global__ void k_totals(const int *__restrict__ a, const int *__restrict__ b, const int *__restrict__ c, int n,
+                         int *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = a[n];
+    out[1] = b[n];
+    out[2] = c[n];
+  }
 }
 
 __global__ void k_permute_pairs(const int *__restrict__ order, int64_t n, const int *__restrict__ src_in,
@@ -133,9 +174,9 @@ __global__ void k_permute_pairs(const int *__restrict__ order, int64_t n, const 
   img_out[i] = img_in[s];
 }
 
-__global__ void k_flag_nonzero(const int *__restrict__ cnt, int n, int *__restrict__ flag) {
+__global__ void k_flag_nonzero(const int *__restrict__ off, int n, int *__restrict__ flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= n) flag[i] = i < n ? (cnt[i] > 0) : 0;
+  if (i <= n) flag[i] = i < n ? (off[i + 1] > off[i]) : 0;
 }
 
 __global__ void k_compact(const int *__restrict__ flag, const int *__restrict__ scan, int n, int *__restrict__ out) {
@@ -177,27 +218,26 @@ static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &sr
   h.ia.reserve(np + 1);
   h.ib.reserve(np + 1);
   h.ic.reserve(np + 1);
-  h.ifl.reserve(nc + 1);
-  EXA_HIP(hipMemsetAsync(h.ifl.p, 0, (nc + 1) * sizeof(int), h.stream));
+  h.id.reserve(np + 1);
+  const int *keys = h.ic.p;
   if (np > 0) {
-    hipLaunchKernelGGL(k_count_targets, dim3(div_up(np, T)), dim3(T), 0, h.stream, tgt.p, np, h.ifl.p);
     hipLaunchKernelGGL(k_seq, dim3(div_up(np, T)), dim3(T), 0, h.stream, h.ia.p, np);
     int bits = 1;
     while ((1 << bits) < nc) bits++;
     bool alt = false;
-    // keys: a copy of tgt (sorted in place), values: emission index.
+    // Keys: a copy of the targets (sorted stably), values: emission index.
     EXA_HIP(hipMemcpyAsync(h.ic.p, tgt.p, np * sizeof(int), hipMemcpyDeviceToDevice, h.stream));
-    h.id.reserve(np + 1);
     sort_pairs_int_int(h, h.ic.p, h.id.p, h.ia.p, h.ib.p, (int)np, bits, alt);
     const int *order = alt ? h.ib.p : h.ia.p;
+    keys = alt ? h.id.p : h.ic.p;
     hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, h.stream, order, np, src.p, img.p, src_out.p,
                        img_out.p);
   }
-  exclusive_scan_int(h, h.ifl.p, off.p, nc + 1);
+  hipLaunchKernelGGL(k_csr_offsets, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, keys, np, nc, off.p);
   // Compact non-empty targets.
   h.ie.reserve(2 * (size_t)nc + 2);
   int *flag = h.ie.p, *scan = h.ie.p + nc + 1;
-  hipLaunchKernelGGL(k_flag_nonzero, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, h.ifl.p, nc, flag);
+  hipLaunchKernelGGL(k_flag_nonzero, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, off.p, nc, flag);
   exclusive_scan_int(h, flag, scan, nc + 1);
   tlist.resize(nc);
   hipLaunchKernelGGL(k_compact, dim3(div_up(nc, T)), dim3(T), 0, h.stream, flag, scan, nc, tlist.p);
@@ -208,15 +248,24 @@ static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &sr
 void run_lists(Handle &h) {
   const int T = 256;
   CellView cv{h.cX.p, h.cChild.p, h.cNChild.p, h.cNBody.p};
-  // Raw (emission-order) lists.
-  DBuf<int> mi, mj, pi, pj;
-  DBuf<unsigned char> mimg, pimg;
+  // Raw (emission-order) lists, kept on the handle across evaluations.
+  DBuf<int> &mi = h.raw_mi, &mj = h.raw_mj, &pi = h.raw_pi, &pj = h.raw_pj;
+  DBuf<unsigned char> &mimg = h.raw_mimg, &pimg = h.raw_pimg;
   int64_t nm = 0, np = 0;
   int nf = h.cfg.images == 0 ? 1 : 27;
+  // Periodic images are traversed non-mutually (traversal.h:250-259).
+  const int mutual = (h.cfg.mutual && h.cfg.images == 0) ? 1 : 0;
   h.fa.reserve(64);
   hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
-  int64_t body_pairs = 0;
-  (void)body_pairs;
+  // Start the raw lists at a capacity that covers uniform trees without
+  // regrowth (about 180 M2L and 65 P2P sources per cell at theta = 0.4).
+  grow_keep(h, mi, (size_t)h.ncells * 128 + 1024, 0);
+  grow_keep(h, mj, (size_t)h.ncells * 128 + 1024, 0);
+  grow_keep(h, mimg, (size_t)h.ncells * 128 + 1024, 0);
+  grow_keep(h, pi, (size_t)h.ncells * 48 + 1024, 0);
+  grow_keep(h, pj, (size_t)h.ncells * 48 + 1024, 0);
+  grow_keep(h, pimg, (size_t)h.ncells * 48 + 1024, 0);
+  h.cnt.reserve(4);
   while (nf > 0) {
     h.ia.reserve(nf + 1);  // action
     h.ib.reserve(3 * ((size_t)nf + 1));  // counts
@@ -225,14 +274,13 @@ void run_lists(Handle &h) {
     int *cm = h.ib.p, *cp = h.ib.p + (nf + 1), *cn = h.ib.p + 2 * (nf + 1);
     int *om = h.ic.p, *op = h.ic.p + (nf + 1), *on = h.ic.p + 2 * (nf + 1);
     hipLaunchKernelGGL(k_classify, dim3(div_up(nf + 1, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
-                       h.cfg.nspawn, act, cm, cp, cn);
+                       h.cfg.nspawn, mutual, act, cm, cp, cn);
     exclusive_scan_int(h, cm, om, nf + 1);
     exclusive_scan_int(h, cp, op, nf + 1);
     exclusive_scan_int(h, cn, on, nf + 1);
     int tot[3];
-    EXA_HIP(hipMemcpyAsync(&tot[0], om + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipMemcpyAsync(&tot[1], op + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipMemcpyAsync(&tot[2], on + nf, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, h.stream, om, op, on, nf, h.cnt.p);
+    EXA_HIP(hipMemcpyAsync(tot, h.cnt.p, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
     grow_keep(h, mi, nm + tot[0] + 1, nm);
     grow_keep(h, mj, nm + tot[0] + 1, nm);
@@ -241,7 +289,8 @@ void run_lists(Handle &h) {
     grow_keep(h, pj, np + tot[1] + 1, np);
     grow_keep(h, pimg, np + tot[1] + 1, np);
     h.fb.reserve((size_t)tot[2] + 1);
-    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, act, om, op, on, nm, np, mi.p,
+    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, mutual, act, om, op, on, nm,
+                       np, mi.p,
                        mj.p, mimg.p, pi.p, pj.p, pimg.p, h.fb.p);
     nm += tot[0];
     np += tot[1];
@@ -253,12 +302,6 @@ void run_lists(Handle &h) {
   h.np2p = np;
   group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt);
   group_by_target(h, np, pi, pj, pimg, h.p2p_off, h.p2p_src, h.p2p_img, h.p2p_tgt, h.np2p_tgt);
-  mi.release();
-  mj.release();
-  pi.release();
-  pj.release();
-  mimg.release();
-  pimg.release();
   EXA_HIP(hipGetLastError());
 }
 

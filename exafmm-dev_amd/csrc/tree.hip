@@ -48,17 +48,27 @@ __global__ void __launch_bounds__(256) k_bounds_partial(const double4 *__restric
 }
 
 __global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
-  if (threadIdx.x == 0) {
-    double r[6];
-    for (int d = 0; d < 6; d++) r[d] = part[d];
-    for (int i = 1; i < nb; i++)
+  __shared__ double s[6][256];
+  double r[6];
+  for (int d = 0; d < 6; d++) r[d] = part[d];
+  for (int i = threadIdx.x; i < nb; i += 256)
+    for (int d = 0; d < 3; d++) {
+      const double a = part[6 * i + d], c = part[6 * i + 3 + d];
+      r[d] = a < r[d] ? a : r[d];
+      r[3 + d] = c > r[3 + d] ? c : r[3 + d];
+    }
+  for (int d = 0; d < 6; d++) s[d][threadIdx.x] = r[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
       for (int d = 0; d < 3; d++) {
-        const double a = part[6 * i + d], c = part[6 * i + 3 + d];
-        r[d] = a < r[d] ? a : r[d];
-        r[3 + d] = c > r[3 + d] ? c : r[3 + d];
+        const double a = s[d][threadIdx.x + w], c = s[3 + d][threadIdx.x + w];
+        s[d][threadIdx.x] = a < s[d][threadIdx.x] ? a : s[d][threadIdx.x];
+        s[3 + d][threadIdx.x] = c > s[3 + d][threadIdx.x] ? c : s[3 + d][threadIdx.x];
       }
-    for (int d = 0; d < 6; d++) out[d] = r[d];
+    __syncthreads();
   }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 void run_bounds(Handle &h) {
@@ -70,7 +80,7 @@ void run_bounds(Handle &h) {
   const int nb = std::min(kBoundsBlocks, div_up(h.n, 256));
   h.red.reserve(6 * nb + 6);
   hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, h.stream, h.in4.p, h.n, h.red.p);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, h.stream, h.red.p, nb, h.red.p + 6 * nb);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, h.stream, h.red.p, nb, h.red.p + 6 * nb);
   double r[6];
   EXA_HIP(hipMemcpyAsync(r, h.red.p + 6 * nb, sizeof(r), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));

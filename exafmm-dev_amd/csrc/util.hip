@@ -34,6 +34,12 @@ Handle::~Handle() {
   p2p_tgt.release();
   m2l_tgt.release();
   m2l_sched.release();
+  raw_mi.release();
+  raw_mj.release();
+  raw_pi.release();
+  raw_pj.release();
+  raw_mimg.release();
+  raw_pimg.release();
   tmp.release();
   k64a.release();
   k64b.release();

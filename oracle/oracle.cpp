@@ -476,38 +476,45 @@ struct orc_ctx {
     }
   }
 
-  void M2L(int ci, int cj, const double *Xp) {  // :149-191 (non-mutual)
+  void M2L(int ci, int cj, const double *Xp, bool mutual) {  // :149-191
     const int P = p.P;
-    std::vector<cd> Y(P * P);
+    std::vector<cd> Y(P * P), Yj(P * P);
     const OCell &Ci = cells[ci], &Cj = cells[cj];
     double dX[3];
     for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - Cj.X[d] - Xp[d];
     double rho, alpha, beta;
     cart2sph(rho, alpha, beta, dX);
     singular_harmonics(P, rho, alpha, beta, Y.data());
+    if (mutual) singular_harmonics(P, rho, alpha + M_PI, beta, Yj.data());
+    const cd *Mi = Mc(ci);
     const cd *Mj = Mc(cj);
     cd *Li = Lc(ci);
+    cd *Lj = Lc(cj);
     for (int j = 0; j < P; j++) {
-      const double Cnm = p.mass ? std::real(Mc(ci)[0] * Mj[0]) * oddeven(j) : oddeven(j);
+      const double Cnm = p.mass ? std::real(Mi[0] * Mj[0]) * oddeven(j) : oddeven(j);
       for (int k = 0; k <= j; k++) {
-        cd acc = 0;
+        cd acc = 0, accj = 0;
         int n0 = 0;
         if (p.mass) {  // :157-169
           acc += Cnm * Y[j * j + j - k];
+          if (mutual) accj += Cnm * Yj[j * j + j - k];
           n0 = 1;
         }
         for (int nn = n0; nn < P - j; nn++) {
           for (int m = -nn; m < 0; m++) {
             const int jnkm = (j + nn) * (j + nn) + j + nn + m - k;
             acc += std::conj(Mj[nn * (nn + 1) / 2 - m]) * Cnm * Y[jnkm];
+            if (mutual) accj += std::conj(Mi[nn * (nn + 1) / 2 - m]) * Cnm * Yj[jnkm];
           }
           for (int m = 0; m <= nn; m++) {
             const int jnkm = (j + nn) * (j + nn) + j + nn + m - k;
             const double Cnm2 = Cnm * oddeven((k - m) * (k < m) + m);
             acc += Mj[nn * (nn + 1) / 2 + m] * Cnm2 * Y[jnkm];
+            if (mutual) accj += Mi[nn * (nn + 1) / 2 + m] * Cnm2 * Yj[jnkm];
           }
         }
         Li[j * (j + 1) / 2 + k] += acc;
+        if (mutual) Lj[j * (j + 1) / 2 + k] += accj;
       }
     }
   }
@@ -585,8 +592,8 @@ struct orc_ctx {
     }
   }
 
-  // LaplaceP2PCPU.cxx:141-169 (scalar FP64 pair kernel, non-mutual).
-  void P2P(int ci, int cj, const double *Xp) {
+  // LaplaceP2PCPU.cxx:141-169 (scalar FP64 pair kernel).
+  void P2P(int ci, int cj, const double *Xp, bool mutual) {
     const OCell &Ci = cells[ci], &Cj = cells[cj];
     for (int i = Ci.ibody; i < Ci.ibody + Ci.nbody; i++) {
       double pot = 0, ax = 0, ay = 0, az = 0;
@@ -598,10 +605,18 @@ struct orc_ctx {
           const double invR2 = 1.0 / R2;
           const double invR = Q[i] * Q[j] * std::sqrt(invR2);
           const double s = invR2 * invR;
+          for (int d = 0; d < 3; d++) dX[d] *= s;
           pot += invR;
-          ax += dX[0] * s;
-          ay += dX[1] * s;
-          az += dX[2] * s;
+          ax += dX[0];
+          ay += dX[1];
+          az += dX[2];
+          if (mutual) {
+            double *Tj = &trg[4 * (size_t)j];
+            Tj[0] += invR;
+            Tj[1] += dX[0];
+            Tj[2] += dX[1];
+            Tj[3] += dX[2];
+          }
         }
       }
       double *T = &trg[4 * (size_t)i];
@@ -705,74 +720,85 @@ struct orc_ctx {
   }
 
   // -------------------------------------------------------------------------
-  // Dual tree traversal, traversal.h:37-124,188-264 (non-mutual).  The
-  // recursion order of TraverseRange (:81-123) is kept, so kernels are
-  // applied in the same order as the reference.
+  // Dual tree traversal, traversal.h:37-124,188-264.  The recursion order of
+  // TraverseRange (:81-123) is kept, so kernels are applied in the same order
+  // as the reference.  A mutual visit of (Ci, Cj), Ci != Cj, is recorded in
+  // the lists as both directed pairs.
   double Xp_cur[3] = {0, 0, 0};
   int img_cur = 13;
 
-  void emit_m2l(int ci, int cj) {
+  void emit_m2l(int ci, int cj, bool mutual) {
     m2l_i.push_back(ci);
     m2l_j.push_back(cj);
     m2l_img.push_back(img_cur);
-    M2L(ci, cj, Xp_cur);
+    if (mutual && ci != cj) {
+      m2l_i.push_back(cj);
+      m2l_j.push_back(ci);
+      m2l_img.push_back(img_cur);
+    }
+    M2L(ci, cj, Xp_cur, mutual);
   }
-  void emit_p2p(int ci, int cj, bool self) {
+  void emit_p2p(int ci, int cj, bool self, bool mutual) {
     p2p_i.push_back(ci);
     p2p_j.push_back(cj);
     p2p_img.push_back(img_cur);
+    if (mutual && ci != cj) {
+      p2p_i.push_back(cj);
+      p2p_j.push_back(ci);
+      p2p_img.push_back(img_cur);
+    }
     if (self)
       P2P_self(ci);
     else
-      P2P(ci, cj, Xp_cur);
+      P2P(ci, cj, Xp_cur, mutual);
   }
 
-  void traverse(int ci, int cj) {
+  void traverse(int ci, int cj, bool mutual) {
     const OCell &Ci = cells[ci], &Cj = cells[cj];
     double dX[3];
     for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - Cj.X[d] - Xp_cur[d];
     const double R2 = norm3(dX);
     if (R2 > (Ci.R + Cj.R) * (Ci.R + Cj.R)) {
-      emit_m2l(ci, cj);
+      emit_m2l(ci, cj, mutual);
     } else if (Ci.nchild == 0 && Cj.nchild == 0) {
       if (Cj.nbody == 0)
-        emit_m2l(ci, cj);
+        emit_m2l(ci, cj, mutual);
       else
-        emit_p2p(ci, cj, R2 == 0 && ci == cj);
+        emit_p2p(ci, cj, R2 == 0 && ci == cj, mutual);
     } else {
-      split(ci, cj);
+      split(ci, cj, mutual);
     }
   }
 
-  void traverse_range(int ib, int ie, int jb, int je) {  // :81-123, mutual = false
+  void traverse_range(int ib, int ie, int jb, int je, bool mutual) {  // :81-123
     if (ie - ib == 1 || je - jb == 1) {
       if (ib == jb) {
-        traverse(ib, jb);
+        traverse(ib, jb, mutual);
       } else {
         for (int i = ib; i < ie; i++)
-          for (int j = jb; j < je; j++) traverse(i, j);
+          for (int j = jb; j < je; j++) traverse(i, j, mutual);
       }
     } else {
       const int im = ib + (ie - ib) / 2, jm = jb + (je - jb) / 2;
-      traverse_range(ib, im, jb, jm);
-      traverse_range(im, ie, jm, je);
-      traverse_range(ib, im, jm, je);
-      traverse_range(im, ie, jb, jm);
+      traverse_range(ib, im, jb, jm, mutual);
+      traverse_range(im, ie, jm, je, mutual);
+      traverse_range(ib, im, jm, je, mutual);
+      if (!mutual || ib != jb) traverse_range(im, ie, jb, jm, mutual);
     }
   }
 
-  void split(int ci, int cj) {  // :188-212
+  void split(int ci, int cj, bool mutual) {  // :188-212
     const OCell &Ci = cells[ci], &Cj = cells[cj];
     if (Cj.nchild == 0) {
-      for (int c = Ci.ichild; c < Ci.ichild + Ci.nchild; c++) traverse(c, cj);
+      for (int c = Ci.ichild; c < Ci.ichild + Ci.nchild; c++) traverse(c, cj, mutual);
     } else if (Ci.nchild == 0) {
-      for (int c = Cj.ichild; c < Cj.ichild + Cj.nchild; c++) traverse(ci, c);
-    } else if (Ci.nbody + Cj.nbody >= p.nspawn) {
-      traverse_range(Ci.ichild, Ci.ichild + Ci.nchild, Cj.ichild, Cj.ichild + Cj.nchild);
+      for (int c = Cj.ichild; c < Cj.ichild + Cj.nchild; c++) traverse(ci, c, mutual);
+    } else if (Ci.nbody + Cj.nbody >= p.nspawn || (mutual && ci == cj)) {
+      traverse_range(Ci.ichild, Ci.ichild + Ci.nchild, Cj.ichild, Cj.ichild + Cj.nchild, mutual);
     } else if (Ci.R >= Cj.R) {
-      for (int c = Ci.ichild; c < Ci.ichild + Ci.nchild; c++) traverse(c, cj);
+      for (int c = Ci.ichild; c < Ci.ichild + Ci.nchild; c++) traverse(c, cj, mutual);
     } else {
-      for (int c = Cj.ichild; c < Cj.ichild + Cj.nchild; c++) traverse(ci, c);
+      for (int c = Cj.ichild; c < Cj.ichild + Cj.nchild; c++) traverse(ci, c, mutual);
     }
   }
 
@@ -793,7 +819,7 @@ struct orc_ctx {
               for (int cy = -1; cy <= 1; cy++)
                 for (int cz = -1; cz <= 1; cz++) {
                   const double Xp[3] = {(ix * 3 + cx) * cycle, (iy * 3 + cy) * cycle, (iz * 3 + cz) * cycle};
-                  m2l_generic(0, cells[0].X, Cp.X, Mp.data(), Xp);
+                  m2l_generic(0, cells[0].X, Cp.X, Mp.data(), Xp);  // M2L(Ci0, Ci, Xp, false)
                 }
           }
       if (p.mass)
@@ -885,7 +911,7 @@ struct orc_ctx {
     if (p.images == 0) {
       for (int d = 0; d < 3; d++) Xp_cur[d] = 0;
       img_cur = 13;
-      traverse(0, 0);
+      traverse(0, 0, p.mutual != 0);
     } else {
       for (int ix = -1; ix <= 1; ix++)
         for (int iy = -1; iy <= 1; iy++)
@@ -894,7 +920,7 @@ struct orc_ctx {
             Xp_cur[1] = iy * p.cycle;
             Xp_cur[2] = iz * p.cycle;
             img_cur = (ix + 1) * 9 + (iy + 1) * 3 + (iz + 1);
-            traverse(0, 0);
+            traverse(0, 0, false);  // images are traversed non-mutually (traversal.h:250-259)
           }
       traverse_periodic();
     }

@@ -31,6 +31,7 @@ typedef struct {
   int adaptive;   /* 1: build_tree_tbb.h semantics, 0: build_tree_omp.h      */
   int mass;       /* 1: MASS kernel variants (all charges positive masses)   */
   int threads;    /* OpenMP threads for the evaluation (0 = default)         */
+  int mutual;     /* mutual traversal and kernels (traversal.h:199-212)      */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

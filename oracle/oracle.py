@@ -25,6 +25,7 @@ class Params(ctypes.Structure):
         ("adaptive", ctypes.c_int),
         ("mass", ctypes.c_int),
         ("threads", ctypes.c_int),
+        ("mutual", ctypes.c_int),
     ]
 
 
@@ -75,9 +76,9 @@ class Oracle:
     STAGES_ALL = 31
 
     def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, images=0,
-                 cycle=2 * np.pi, adaptive=False, mass=False, threads=0):
+                 cycle=2 * np.pi, adaptive=False, mass=False, threads=0, mutual=False):
         self.params = Params(P, theta, ncrit, nspawn, useRmax, useRopt, images, cycle, int(adaptive), int(mass),
-                             threads)
+                             threads, int(mutual))
         self.h = lib().orc_create(ctypes.byref(self.params))
         self.P = P
 

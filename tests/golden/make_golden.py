@@ -40,6 +40,12 @@ CASES = [
     ("unit16k_u_p10", "harness_u_p10", ["file"], [0.4, 64, 1000, 1, 0, 0, 0], "tree"),
     ("small100_u_p10", "harness_u_p10", ["gen", "c", "100"], [0.4, 64, 1000, 1, 0, 0, 0], "full"),
     ("tiny40_u_p10", "harness_u_p10", ["gen", "c", "40"], [0.4, 64, 1000, 1, 0, 0, 0], "full"),
+    # Mutual traversal (the reference's default -m 1).
+    ("cube4k_u_p10_mut", "harness_u_p10", ["gen", "c", "4096"], [0.4, 64, 1000, 1, 0, 1, 0], "full"),
+    ("cube16k_u_p10_mut", "harness_u_p10", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
+    ("plummer16k_u_p10_mut", "harness_u_p10", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
+    ("cube16k_a_p10_mut", "harness_a_p10", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
+    ("mass_plummer16k_a_p10_mut", "harness_a_p10_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 1, 0], "tree"),
 ]
 
 
@@ -71,7 +77,7 @@ def main():
             r = O.read_ref_dump(out)
             c = r["cells"]
             d = dict(theta=opts[0], ncrit=opts[1], nspawn=opts[2], useRmax=opts[3],
-                     useRopt=opts[4], images=opts[6], P=r["P"], num_p2p=r["num_p2p"], num_m2l=r["num_m2l"],
+                     useRopt=opts[4], mutual=opts[5], images=opts[6], P=r["P"], num_p2p=r["num_p2p"], num_m2l=r["num_m2l"],
                      err_pot=r["err_pot"], err_acc=r["err_acc"], xmin=r["xmin"], xmax=r["xmax"],
                      ibody=r["ibody"], key=r["key"], sample_ibody=r["sample_ibody"],
                      sample_direct=r["sample_direct"], adaptive="_a_" in name, mass=name.startswith("mass"),

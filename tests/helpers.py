@@ -19,7 +19,7 @@ def load_case(name):
 def case_params(z):
     return dict(P=int(z["P"]), theta=float(z["theta"]), ncrit=int(z["ncrit"]), nspawn=int(z["nspawn"]),
                 useRmax=int(z["useRmax"]), useRopt=int(z["useRopt"]), images=int(z["images"]),
-                adaptive=bool(z["adaptive"]), mass=bool(z["mass"]))
+                adaptive=bool(z["adaptive"]), mass=bool(z["mass"]), mutual=bool(z["mutual"]))
 
 
 def unit_cube(n, seed):

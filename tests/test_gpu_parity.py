@@ -14,14 +14,16 @@ from helpers import case_params, load_case, rel_l2, run_oracle, unit_cube
 pytestmark = pytest.mark.gpu
 
 UNIFORM = ["cube4k_u_p10", "cube16k_u_p10", "plummer16k_u_p10", "unit16k_u_p10", "cube4k_u_p4",
-           "mass_plummer8k_u_p8", "small100_u_p10", "tiny40_u_p10"]
+           "mass_plummer8k_u_p8", "small100_u_p10", "tiny40_u_p10", "cube4k_u_p10_mut", "cube16k_u_p10_mut",
+           "plummer16k_u_p10_mut"]
 
 
 def gpu_fmm(z, builder="uniform"):
     p = case_params(z)
-    print(f"[gpu] N={z['input'].shape[0]} P={p['P']} builder={builder} mass={p['mass']}", flush=True)
+    print(f"[gpu] N={z['input'].shape[0]} P={p['P']} builder={builder} mass={p['mass']} mutual={p['mutual']}",
+          flush=True)
     f = X.FMM(P=p["P"], theta=p["theta"], ncrit=p["ncrit"], nspawn=p["nspawn"], useRmax=p["useRmax"],
-              useRopt=p["useRopt"], images=p["images"], builder=builder, mass=p["mass"])
+              useRopt=p["useRopt"], images=p["images"], builder=builder, mass=p["mass"], mutual=int(p["mutual"]))
     f.set_bodies(z["input"])
     return f
 
@@ -58,7 +60,10 @@ def test_uniform_fmm_matches_oracle(name):
         g = np.sort(gl[a].astype(np.int64) * 10 ** 7 + gl[b])
         r = np.sort(ol[a].astype(np.int64) * 10 ** 7 + ol[b])
         assert np.array_equal(g, r), a
-    assert len(gl["m2l_i"]) == z["num_m2l"] and len(gl["p2p_i"]) == z["num_p2p"]
+    if not p["mutual"]:
+        assert len(gl["m2l_i"]) == z["num_m2l"] and len(gl["p2p_i"]) == z["num_p2p"]
+    else:
+        assert len(gl["m2l_i"]) == 2 * z["num_m2l"]
     c, oc = f.cells(), o.cells()
     scale = np.abs(oc["M"]).max()
     assert np.abs(c["M"] - oc["M"]).max() <= 1e-12 * scale

@@ -13,7 +13,9 @@ from helpers import case_params, load_case, rel_l2
 
 CASES = ["cube4k_u_p10", "cube16k_u_p10", "cube16k_a_p10", "plummer16k_a_p10", "plummer16k_u_p10",
          "sphere16k_a_p8", "mass_plummer16k_a_p10", "mass_plummer8k_u_p8", "cube4k_u_p4", "cube4k_a_p12",
-         "octant8k_a_p6", "cube2k_u_p6_periodic", "unit16k_u_p10", "small100_u_p10", "tiny40_u_p10"]
+         "octant8k_a_p6", "cube2k_u_p6_periodic", "unit16k_u_p10", "small100_u_p10", "tiny40_u_p10",
+         "cube4k_u_p10_mut", "cube16k_u_p10_mut", "plummer16k_u_p10_mut", "cube16k_a_p10_mut",
+         "mass_plummer16k_a_p10_mut"]
 
 
 def test_drand48_and_generators(golden_dir):
@@ -53,9 +55,14 @@ def test_oracle_matches_reference(name):
     if not z["adaptive"]:
         assert np.array_equal(b["key"], z["key"])
     lst = o.lists()
-    if int(z["images"]) == 0:
+    if int(z["images"]) == 0 and not z["mutual"]:
         assert len(lst["p2p_i"]) == z["num_p2p"]
         assert len(lst["m2l_i"]) == z["num_m2l"]
+    elif int(z["images"]) == 0:
+        # A mutual call covers both directions; each leaf's self pair is one call.
+        nleaf = int(np.sum(c["nchild"] == 0))
+        assert len(lst["m2l_i"]) == 2 * z["num_m2l"]
+        assert len(lst["p2p_i"]) == 2 * z["num_p2p"] - nleaf
     # Results of the sampled bodies.
     pos = np.empty(len(b["ibody"]), np.int64)
     pos[b["ibody"]] = np.arange(len(b["ibody"]))
