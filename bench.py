@@ -147,7 +147,8 @@ def main():
         "p2p_ginter_per_s": p2p_pairs / (t_p2p * 1e-3) / 1e9,
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
         "phases_ms": {k: round(med(k), 4) for k in
-                      ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_downward", "t_total")},
+                      ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_traverse", "t_downward",
+                       "t_total")},
         "counts": {"cells": s["num_cells"], "p2p_cell_pairs": s["num_p2p_pairs"],
                    "m2l_cell_pairs": s["num_m2l_pairs"], "p2p_body_pairs": p2p_pairs},
         "kernels_tflops": {"m2l": round(m2l_tf, 4), "p2p": round(p2p_tf, 4)},

@@ -85,15 +85,23 @@ static void do_upward(Handle &h) {
   h.have_up = true;
 }
 
+// Lists on the main stream; then P2P on the second stream concurrently with
+// M2L on the main stream (independent outputs: body results vs local
+// expansions); the main stream waits for P2P before anything downstream.
 static void do_traverse(Handle &h) {
   ensure(h.have_up, "traverse: upward pass not run");
   EXA_HIP(hipEventRecord(h.ev[4], h.stream));
   run_lists(h);
   EXA_HIP(hipEventRecord(h.ev[5], h.stream));
-  run_p2p(h);
+  EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[5], 0));
+  EXA_HIP(hipEventRecord(h.ev[9], h.stream2));
+  run_p2p(h, h.stream2);
+  EXA_HIP(hipEventRecord(h.ev[10], h.stream2));
   EXA_HIP(hipEventRecord(h.ev[6], h.stream));
   run_m2l(h);
   if (h.cfg.images > 0) run_periodic(h);
+  EXA_HIP(hipEventRecord(h.ev[11], h.stream));
+  EXA_HIP(hipStreamWaitEvent(h.stream, h.ev[10], 0));
   EXA_HIP(hipEventRecord(h.ev[7], h.stream));
   h.have_trav = true;
 }
@@ -147,6 +155,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   try {
     set_device(H->h);
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
+    EXA_HIP(hipStreamCreateWithFlags(&H->h.stream2, hipStreamNonBlocking));
     for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));
   } catch (const Error &e) {
     delete H;
@@ -252,8 +261,9 @@ int exafmm_hip_evaluate(exafmm_hip_handle *H) {
     s.t_build = elapsed(h.ev[1], h.ev[2]);
     s.t_upward = elapsed(h.ev[2], h.ev[3]);
     s.t_lists = elapsed(h.ev[4], h.ev[5]);
-    s.t_p2p = elapsed(h.ev[5], h.ev[6]);
-    s.t_m2l = elapsed(h.ev[6], h.ev[7]);
+    s.t_p2p = elapsed(h.ev[9], h.ev[10]);   // on the P2P stream
+    s.t_m2l = elapsed(h.ev[6], h.ev[11]);   // on the main stream
+    s.t_traverse = elapsed(h.ev[4], h.ev[7]);
     s.t_downward = elapsed(h.ev[7], h.ev[8]);
     s.t_total = elapsed(h.ev[0], h.ev[8]);
   });
@@ -379,7 +389,10 @@ int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
 }
 
 int exafmm_hip_synchronize(exafmm_hip_handle *H) {
-  return guarded(H, [&](Handle &h) { EXA_HIP(hipStreamSynchronize(h.stream)); });
+  return guarded(H, [&](Handle &h) {
+    EXA_HIP(hipStreamSynchronize(h.stream2));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+  });
 }
 
 }  // extern "C"

@@ -68,6 +68,7 @@ struct Handle {
   int nterm = 0;
   std::string err;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // P2P runs here, concurrently with M2L
   hipEvent_t ev[16] = {};
 
   // Bodies.
@@ -125,7 +126,7 @@ void run_build_uniform(Handle &h);
 void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
 void run_lists(Handle &h);
-void run_p2p(Handle &h);
+void run_p2p(Handle &h, hipStream_t st);
 void run_m2l(Handle &h);
 void run_downward(Handle &h);
 void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);

@@ -123,8 +123,10 @@ __global__ void __launch_bounds__(64)
   constexpr int NF = S::NF;
   constexpr int IPL = S::IPL;
   constexpr int NO = (NT + 63) / 64;
-  __shared__ cplx sM[2][NF + 1];
-  __shared__ cplx sS[2][NF + 1];
+  // Single-buffered tables (two barriers per pair) keep the LDS footprint
+  // small enough for five or more resident waves per SIMD.
+  __shared__ cplx sM[NF + 1];
+  __shared__ cplx sS[NF + 1];
   __shared__ cplx seg[S::NTRIPLE];
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
@@ -134,8 +136,8 @@ __global__ void __launch_bounds__(64)
   const int myslot0 = slot0[lane];
   for (int t = lane; t < S::NTRIPLE; t += 64) seg[t] = {0, 0};
   if (lane == 0) {
-    sM[0][NF] = sM[1][NF] = {0, 0};
-    sS[0][NF] = sS[1][NF] = {0, 0};
+    sM[NF] = {0, 0};
+    sS[NF] = {0, 0};
   }
   // Packed coefficients (n, m >= 0) this lane translates: t = lane + 64 o.
   int tn[NO], tm[NO];
@@ -159,7 +161,6 @@ __global__ void __launch_bounds__(64)
   double4 X_next = cX[s_next];
   double mj0_next = mass ? M[(size_t)s_next * NT].re : 1.0;
   for (int e = e0; e < e1; e++) {
-    const int buf = (e - e0) & 1;
     const double4 Xj = X_next;
     const int im = im_next;
     cplx mcur[NO];
@@ -183,6 +184,7 @@ __global__ void __launch_bounds__(64)
     const double ct = dz * u, su = rxy2 * w * u * u;  // cos(alpha), sin(alpha) / r
     const double cp = rxy2 > 0 ? dx * w : 1.0, sp = dy * w;
     const double cu = ct * u, u2 = u * u;
+    __syncthreads();  // the previous pair's items are done with the tables
 #pragma unroll
     for (int o = 0; o < NO; o++) {
       if (lane + 64 * o >= NT) continue;
@@ -205,18 +207,18 @@ __global__ void __launch_bounds__(64)
       const double sr = g * ec, si = g * es;
       // S'_a^b = (-1)^b S_a^b for b > 0, S'_a^{-b} = conj(S_a^b).
       const double sg = (b & 1) ? -1.0 : 1.0;
-      sS[buf][a * a + a + b] = {sg * sr, sg * si};
-      if (b > 0) sS[buf][a * a + a - b] = {sr, -si};
+      sS[a * a + a + b] = {sg * sr, sg * si};
+      if (b > 0) sS[a * a + a - b] = {sr, -si};
       // M'_n^m = (-1)^m M_n^m (m >= 0), M'_n^{-m} = conj(M_n^m).
       cplx mv = mcur[o];
       if (mass && a == 0) mv = {1.0, 0.0};
-      sM[buf][a * a + a + b] = {sg * mv.re, sg * mv.im};
-      if (b > 0) sM[buf][a * a + a - b] = {mv.re, -mv.im};
+      sM[a * a + a + b] = {sg * mv.re, sg * mv.im};
+      if (b > 0) sM[a * a + a - b] = {mv.re, -mv.im};
     }
     const double scale = mass ? mi0 * mj0 : 1.0;
     __syncthreads();
-    const char *mb = reinterpret_cast<const char *>(sM[buf]);
-    const char *sb = reinterpret_cast<const char *>(sS[buf]);
+    const char *mb = reinterpret_cast<const char *>(sM);
+    const char *sb = reinterpret_cast<const char *>(sS);
     double ar = 0, ai = 0;
     int slot = myslot0;
 #pragma unroll
@@ -224,8 +226,10 @@ __global__ void __launch_bounds__(64)
       const uint32_t d = desc[it];
       const cplx mv = *reinterpret_cast<const cplx *>(mb + (d & 0xFFF));
       const cplx sv = *reinterpret_cast<const cplx *>(sb + ((d >> 12) & 0xFFF));
-      ar += mv.re * sv.re - mv.im * sv.im;
-      ai += mv.re * sv.im + mv.im * sv.re;
+      ar = fma(mv.re, sv.re, ar);
+      ar = fma(-mv.im, sv.im, ar);
+      ai = fma(mv.re, sv.im, ai);
+      ai = fma(mv.im, sv.re, ai);
       if (d & (1u << 24)) {
         cplx t = seg[slot];
         t.re += scale * ar;

@@ -10,7 +10,7 @@
 // Lane layout: a leaf's targets are replicated over G = 64 / T lane groups
 // (T = 16, 32 or 64 targets per group) and group g takes every G-th source
 // body, so short leaves still fill the wavefront.  Source bodies are staged
-// through LDS up to 256 at a time (consecutive list entries, each shifted by
+// through LDS up to 128 at a time (consecutive list entries, each shifted by
 // its periodic image); the G partial sums are combined in a fixed order at
 // the end (deterministic, no atomics).
 #include "internal.h"
@@ -23,7 +23,7 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
   pz = (img % 3 - 1) * cycle;
 }
 
-constexpr int kP2PStage = 256;  // source bodies staged in LDS per fill
+constexpr int kP2PStage = 128;  // source bodies staged in LDS per fill
 
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
@@ -118,10 +118,10 @@ __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, co
   atomicAdd(out, s * (unsigned long long)cNBody[c]);
 }
 
-void run_p2p(Handle &h) {
-  EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), h.stream));
+void run_p2p(Handle &h, hipStream_t st) {
+  EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), st));
   if (h.np2p_tgt == 0) return;
-  hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, h.stream, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
+  hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
                      h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   EXA_HIP(hipGetLastError());
 }

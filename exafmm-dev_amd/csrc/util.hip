@@ -56,6 +56,7 @@ Handle::~Handle() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
+  if (stream2) (void)hipStreamDestroy(stream2);
 }
 
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n) {

@@ -46,7 +46,8 @@ class Config(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_downward", "t_total")] + [
+                ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_downward", "t_total",
+                 "t_traverse")] + [
         ("num_p2p_pairs", ctypes.c_int64), ("num_m2l_pairs", ctypes.c_int64), ("num_p2p_bodies", ctypes.c_int64),
         ("num_cells", ctypes.c_int), ("num_levels", ctypes.c_int)]
 

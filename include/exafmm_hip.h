@@ -47,9 +47,11 @@ typedef struct {
   int device;     /* HIP device ordinal                                        */
 } exafmm_hip_config;
 
-/* Phase timings (ms, HIP events on the handle's stream) and work counts. */
+/* Phase timings (ms, HIP events) and work counts.  P2P runs on a second
+ * stream concurrently with M2L: t_p2p and t_m2l are each kernel's own span,
+ * t_traverse the whole traversal (lists + overlapped P2P/M2L). */
 typedef struct {
-  double t_bounds, t_build, t_upward, t_lists, t_p2p, t_m2l, t_downward, t_total;
+  double t_bounds, t_build, t_upward, t_lists, t_p2p, t_m2l, t_downward, t_total, t_traverse;
   int64_t num_p2p_pairs;   /* cell pairs in the P2P list                      */
   int64_t num_m2l_pairs;   /* cell pairs in the M2L list                      */
   int64_t num_p2p_bodies;  /* target-source body pairs evaluated by P2P       */
