@@ -398,6 +398,5 @@ int exafmm_hip_synchronize(exafmm_hip_handle *H) {
 }  // extern "C"
 
 namespace exafmm {
-void run_build_adaptive(Handle &) { throw Error(EXAFMM_HIP_EUNSUP, "adaptive builder not available yet"); }
 void run_periodic(Handle &) { throw Error(EXAFMM_HIP_EUNSUP, "periodic images not available yet"); }
 }  // namespace exafmm

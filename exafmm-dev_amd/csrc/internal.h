@@ -101,6 +101,11 @@ struct Handle {
   DBuf<int> raw_mi, raw_mj, raw_pi, raw_pj;  // emission-order lists
   DBuf<unsigned char> raw_mimg, raw_pimg;
 
+  // Adaptive-builder scratch (node arrays in creation order).
+  DBuf<double4> at_X, at_aX, at_body2;
+  DBuf<int> at_start, at_count, at_parent, at_leaf, at_level, at_first, at_nchild, at_astart, at_acount, at_agid,
+      at_idx2, at_nnode, at_cellid, at_cursor, at_nl;
+
   // M2L lane schedule (a pure function of P).
   DBuf<uint32_t> m2l_sched;
   int m2l_sched_P = -1;

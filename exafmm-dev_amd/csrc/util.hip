@@ -34,6 +34,10 @@ Handle::~Handle() {
   p2p_tgt.release();
   m2l_tgt.release();
   m2l_sched.release();
+  for (auto *b : {&at_X, &at_aX, &at_body2}) b->release();
+  for (auto *b : {&at_start, &at_count, &at_parent, &at_leaf, &at_level, &at_first, &at_nchild, &at_astart,
+                  &at_acount, &at_agid, &at_idx2, &at_nnode, &at_cellid, &at_cursor, &at_nl})
+    b->release();
   raw_mi.release();
   raw_mj.release();
   raw_pi.release();

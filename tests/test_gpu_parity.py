@@ -80,6 +80,52 @@ def test_uniform_fmm_matches_oracle(name):
     f.close()
 
 
+ADAPTIVE = ["cube16k_a_p10", "plummer16k_a_p10", "sphere16k_a_p8", "octant8k_a_p6", "cube4k_a_p12",
+            "mass_plummer16k_a_p10", "cube16k_a_p10_mut", "mass_plummer16k_a_p10_mut"]
+
+
+@pytest.mark.parametrize("name", ADAPTIVE)
+def test_adaptive_tree_bitexact(name):
+    z = load_case(name)
+    f = gpu_fmm(z, builder="adaptive")
+    f.get_bounds()
+    f.build_tree()
+    f.upward_pass()
+    c = f.cells()
+    for k in ("iparent", "ichild", "nchild", "ibody", "nbody", "icell"):
+        assert np.array_equal(c[k], z["c_" + k]), k
+    assert np.array_equal(c["X"], z["c_X"])
+    if not z["useRopt"]:
+        assert np.array_equal(c["R"], z["c_R"]), np.abs(c["R"] - z["c_R"]).max()
+    else:  # SetRopt runs on device M[0]: radii agree to rounding
+        assert np.allclose(c["R"], z["c_R"], rtol=1e-12, atol=0)
+    b = f.bodies()
+    assert np.array_equal(b["ibody"], z["ibody"])
+    f.close()
+
+
+@pytest.mark.parametrize("name", ADAPTIVE)
+def test_adaptive_fmm_matches_oracle(name):
+    z = load_case(name)
+    p = case_params(z)
+    o = run_oracle(z["input"], **p)
+    f = gpu_fmm(z, builder="adaptive")
+    f.evaluate()
+    gl, ol = f.lists(), o.lists()
+    for a, b in (("p2p_i", "p2p_j"), ("m2l_i", "m2l_j")):
+        g = np.sort(gl[a].astype(np.int64) * 10 ** 7 + gl[b])
+        r = np.sort(ol[a].astype(np.int64) * 10 ** 7 + ol[b])
+        assert np.array_equal(g, r), a
+    trg = f.results()
+    ob = o.bodies()
+    otrg = np.empty_like(ob["trg"])
+    otrg[ob["ibody"]] = ob["trg"]
+    assert rel_l2(trg[:, 0], otrg[:, 0]) < 1e-12
+    assert rel_l2(trg[:, 1:], otrg[:, 1:]) < 1e-12
+    assert rel_l2(trg[z["sample_ibody"]], z["sample_fmm"]) < 1e-12
+    f.close()
+
+
 @pytest.mark.parametrize("name", ["cube16k_u_p10", "plummer16k_u_p10"])
 def test_gpu_direct_matches_reference(name):
     z = load_case(name)
