@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--P", type=int, default=10)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--with-mixed", action="store_true", default=True)
+    ap.add_argument("--no-mixed", dest="with_mixed", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; multi-GPU sharding is design-only "
@@ -88,40 +90,50 @@ def main():
 
     n, P = args.n, args.P
     x = unit_cube(n, 2024)
-    f = exafmm_hip.FMM(P=P, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, builder="uniform",
-                       precision=args.precision)
-    f.set_bodies(x)  # host -> HBM once; steps read the resident copy
-    for _ in range(args.warmup):
-        f.evaluate()
-    f.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.evaluate()
-        stats.append(f.stats())
-    f.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    value = n / (ms * 1e-3)
-
-    # Accuracy against FP64 direct summation on 128 random targets.
     rng = np.random.default_rng(7)
     tgt = rng.choice(n, 128, replace=False).astype(np.int32)
-    res = f.results()[tgt]
-    ref = f.direct(tgt)
-    err_pot = float(np.sqrt(np.sum((res[:, 0] - ref[:, 0]) ** 2) / np.sum(ref[:, 0] ** 2)))
-    err_acc = float(np.sqrt(np.sum((res[:, 1:] - ref[:, 1:]) ** 2) / np.sum(ref[:, 1:] ** 2)))
 
-    s = stats[-1]
-    med = lambda k: float(np.median([d[k] for d in stats]))  # noqa: E731
-    t_m2l, t_p2p = med("t_m2l"), med("t_p2p")
+    def run(precision):
+        f = exafmm_hip.FMM(P=P, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, builder="uniform",
+                           precision=precision)
+        f.set_bodies(x)  # host -> HBM once; steps read the resident copy
+        for _ in range(args.warmup):
+            f.evaluate()
+        f.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f.evaluate()
+            stats.append(f.stats())
+        f.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        # Accuracy against FP64 direct summation on 128 random targets.
+        res = f.results()[tgt]
+        ref = f.direct(tgt)
+        err_pot = float(np.sqrt(np.sum((res[:, 0] - ref[:, 0]) ** 2) / np.sum(ref[:, 0] ** 2)))
+        err_acc = float(np.sqrt(np.sum((res[:, 1:] - ref[:, 1:]) ** 2) / np.sum(ref[:, 1:] ** 2)))
+        f.close()
+        med = {k: float(np.median([d[k] for d in stats])) for k in stats[0] if k.startswith("t_")}
+        return ms, stats[-1], med, err_pot, err_acc
+
+    ms, s, med, err_pot, err_acc = run("fp64")
+    value = n / (ms * 1e-3)
+    mixed = None
+    if args.precision == "mixed" or args.with_mixed:
+        mms, ms_stats, mmed, mpot, macc = run("mixed")
+        mixed = {"value": n / (mms * 1e-3), "unit": "particles/s", "ms_per_step": mms,
+                 "speedup_vs_fp64": ms / mms, "rel_l2_err": {"pot": mpot, "acc": macc, "targets": 128},
+                 "phases_ms": {k: round(v, 4) for k, v in mmed.items()},
+                 "dtype": "f32 P2P (compensated) and f32 M2L tables; f64 centres, offsets, expansions"}
+
+    t_m2l, t_p2p = med["t_m2l"], med["t_p2p"]
     m2l_flops = s["num_m2l_pairs"] * m2l_flops_per_pair(P)
     p2p_pairs = s["num_p2p_bodies"]
     p2p_flops = 20 * p2p_pairs
     m2l_tf = m2l_flops / (t_m2l * 1e-3) / 1e12
     p2p_tf = p2p_flops / (t_p2p * 1e-3) / 1e12
     if t_m2l >= t_p2p:
-        dom = {"kernel": "k_m2l_f64", "bound": "mfma", "achieved": m2l_tf, "ms": t_m2l}
+        dom = {"kernel": "k_m2l<%d,double>" % P, "bound": "mfma", "achieved": m2l_tf, "ms": t_m2l}
     else:
         dom = {"kernel": "k_p2p_f64", "bound": "mfma", "achieved": p2p_tf, "ms": t_p2p}
     roofline = {"bound": dom["bound"], "achieved": round(dom["achieved"], 4), "peak": PEAK_FP64_TFLOPS,
@@ -140,23 +152,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic (U[-1,1]^3 bodies, zero-mean uniform charges / N, seeded)",
         "config": {"workload": "plain FP64 GPU FMM evaluation, Laplace spherical", "N": n, "P": P, "theta": 0.4,
-                   "ncrit": 64, "tree": "uniform Morton (build_tree_omp.h)", "precision": args.precision},
+                   "ncrit": 64, "tree": "uniform Morton (build_tree_omp.h)", "precision": "fp64"},
         "p2p_ginter_per_s": p2p_pairs / (t_p2p * 1e-3) / 1e9,
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
-        "phases_ms": {k: round(med(k), 4) for k in
-                      ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_traverse", "t_downward",
-                       "t_total")},
+        "phases_ms": {k: round(v, 4) for k, v in med.items()},
         "counts": {"cells": s["num_cells"], "p2p_cell_pairs": s["num_p2p_pairs"],
                    "m2l_cell_pairs": s["num_m2l_pairs"], "p2p_body_pairs": p2p_pairs},
         "kernels_tflops": {"m2l": round(m2l_tf, 4), "p2p": round(p2p_tf, 4)},
         "roofline": roofline,
+        "mixed": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    f.close()
 
 
 if __name__ == "__main__":

@@ -55,6 +55,7 @@ struct M2LShape {
 //   bits [12,24) byte offset of S'_{j+n}^{m-k} in the S' table
 //   bit 24       last item of a triple: add the partial into the lane's next
 //                accumulator slot
+//   bits [25,29) j of the triple (selects the triple's scale factor)
 // Padding items point at the zero entry NF of both tables.  Slots are laid
 // out lane-major (lane l owns slots [slot0[l], slot0[l] + #triples(l))), and
 // `outslot` lists, for every output (j, k), its slots in n order.
@@ -65,7 +66,12 @@ struct M2LSchedule {
   int nslots = 0;
 };
 
-static M2LSchedule build_schedule(int P, int ipl) {
+template <typename R>
+struct cplxT {
+  R re, im;
+};
+
+static M2LSchedule build_schedule(int P, int ipl, int esz) {
   struct Triple {
     int j, k, n, work;
   };
@@ -87,7 +93,7 @@ static M2LSchedule build_schedule(int P, int ipl) {
   }
   const int NF = P * P, NT = P * (P + 1) / 2;
   M2LSchedule s;
-  const uint32_t pad = (uint32_t)(NF * 16) | ((uint32_t)(NF * 16) << 12);
+  const uint32_t pad = (uint32_t)(NF * esz) | ((uint32_t)(NF * esz) << 12);
   s.desc.assign(64 * (size_t)ipl, pad);
   s.slot0.assign(64, 0);
   s.outslot.assign((size_t)NT * P, -1);
@@ -100,8 +106,8 @@ static M2LSchedule build_schedule(int P, int ipl) {
       const Triple &q = tr[t];
       for (int m = -q.n; m <= q.n; m++) {
         const int a = q.j + q.n, b = m - q.k;
-        uint32_t d = (uint32_t)((q.n * q.n + q.n + m) * 16) | ((uint32_t)((a * a + a + b) * 16) << 12);
-        if (m == q.n) d |= 1u << 24;
+        uint32_t d = (uint32_t)((q.n * q.n + q.n + m) * esz) | ((uint32_t)((a * a + a + b) * esz) << 12);
+        if (m == q.n) d |= (1u << 24) | ((uint32_t)q.j << 25);
         s.desc[(size_t)it * 64 + l] = d;
         it++;
       }
@@ -112,22 +118,32 @@ static M2LSchedule build_schedule(int P, int ipl) {
   return s;
 }
 
-template <int P>
+// R = double: the plain FP64 path.  R = float: the mixed path -- centres,
+// the offset geometry and the accumulators stay FP64; the harmonic tables and
+// the per-triple contraction are FP32.  To keep the FP32 tables in range at
+// any cell size, they hold the expansions scaled by the pair distance r:
+//   M~_n^m = M_n^m / r^n,  S~_a^b = S_a^b r^(a+1)  (unit-distance harmonics),
+// so L_j^k = r^-(j+1) sum_n sum_m M~ S~, and each finished (j, k, n) partial
+// is multiplied in FP64 by u^(j+1) = r^-(j+1).
+template <int P, typename R>
 __global__ void __launch_bounds__(64)
-    k_m2l_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
-              const unsigned char *__restrict__ img, const double4 *__restrict__ cX, const cplx *__restrict__ M,
-              cplx *__restrict__ L, const uint32_t *__restrict__ sched, const int *__restrict__ slot0,
-              const int *__restrict__ outslot, double cycle, int mass) {
+    k_m2l(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
+          const unsigned char *__restrict__ img, const double4 *__restrict__ cX, const cplx *__restrict__ M,
+          cplx *__restrict__ L, const uint32_t *__restrict__ sched, const int *__restrict__ slot0,
+          const int *__restrict__ outslot, double cycle, int mass) {
   using S = M2LShape<P>;
+  using C = cplxT<R>;
+  constexpr bool F32 = sizeof(R) == 4;
   constexpr int NT = Expn<P>::NT;
   constexpr int NF = S::NF;
   constexpr int IPL = S::IPL;
   constexpr int NO = (NT + 63) / 64;
   // Single-buffered tables (two barriers per pair) keep the LDS footprint
   // small enough for five or more resident waves per SIMD.
-  __shared__ cplx sM[NF + 1];
-  __shared__ cplx sS[NF + 1];
+  __shared__ C sM[NF + 1];
+  __shared__ C sS[NF + 1];
   __shared__ cplx seg[S::NTRIPLE];
+  __shared__ double sU[P];  // FP32 path: scale * u^(j+1)
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
   uint32_t desc[IPL];
@@ -181,10 +197,17 @@ __global__ void __launch_bounds__(64)
     const double rxy2 = dx * dx + dy * dy;
     const double u = rsqrt(rxy2 + dz * dz);
     const double w = rxy2 > 0 ? rsqrt(rxy2) : 0.0;
-    const double ct = dz * u, su = rxy2 * w * u * u;  // cos(alpha), sin(alpha) / r
+    const double ct = dz * u, st = rxy2 * w * u;  // cos(alpha), sin(alpha)
     const double cp = rxy2 > 0 ? dx * w : 1.0, sp = dy * w;
-    const double cu = ct * u, u2 = u * u;
+    const double us = F32 ? 1.0 : u;  // the FP32 tables use unit distance
+    const double ru = us, rsu = st * us, rcp = cp, rsp = sp, cu = ct * us, u2 = us * us;
+    const double scale = mass ? mi0 * mj0 : 1.0;
     __syncthreads();  // the previous pair's items are done with the tables
+    if (F32 && lane < P) {
+      double f = scale * u;
+      for (int i = 0; i < lane; i++) f *= u;
+      sU[lane] = f;
+    }
 #pragma unroll
     for (int o = 0; o < NO; o++) {
       if (lane + 64 * o >= NT) continue;
@@ -192,48 +215,55 @@ __global__ void __launch_bounds__(64)
       // S_a^b = (-1)^a (a-b)! u^(a+1) P_a^b(ct) e^{ib phi} = G_a e^{ib phi}, with
       // G_b = (2b-1)!! (st u)^b u and the division-free upward recurrence
       // G_a = -(2a-1) ct u G_{a-1} - (a+b-1)(a-b-1) u^2 G_{a-2}.
-      double g = u, gm1 = 0, ec = 1, es = 0;
+      // Built in FP64 for both paths and rounded once when stored.
+      double g = ru, gm1 = 0, ec = 1, es = 0;
       for (int i = 1; i <= b; i++) {
-        g *= su * (2 * i - 1);
-        const double t = ec * cp - es * sp;
-        es = ec * sp + es * cp;
+        g *= rsu * (double)(2 * i - 1);
+        const double t = ec * rcp - es * rsp;
+        es = ec * rsp + es * rcp;
         ec = t;
       }
       for (int t = b + 1; t <= a; t++) {
-        const double gn = -(2 * t - 1) * cu * g - (double)((t + b - 1) * (t - b - 1)) * u2 * gm1;
+        const double gn = -(double)(2 * t - 1) * cu * g - (double)((t + b - 1) * (t - b - 1)) * u2 * gm1;
         gm1 = g;
         g = gn;
       }
-      const double sr = g * ec, si = g * es;
+      const R sr = (R)(g * ec), si = (R)(g * es);
       // S'_a^b = (-1)^b S_a^b for b > 0, S'_a^{-b} = conj(S_a^b).
-      const double sg = (b & 1) ? -1.0 : 1.0;
+      const R sg = (b & 1) ? (R)-1 : (R)1;
       sS[a * a + a + b] = {sg * sr, sg * si};
       if (b > 0) sS[a * a + a - b] = {sr, -si};
       // M'_n^m = (-1)^m M_n^m (m >= 0), M'_n^{-m} = conj(M_n^m).
       cplx mv = mcur[o];
       if (mass && a == 0) mv = {1.0, 0.0};
-      sM[a * a + a + b] = {sg * mv.re, sg * mv.im};
-      if (b > 0) sM[a * a + a - b] = {mv.re, -mv.im};
+      if (F32) {  // M~_a^b = M_a^b u^a
+        double f = 1.0;
+        for (int i = 0; i < a; i++) f *= u;
+        mv.re *= f;
+        mv.im *= f;
+      }
+      sM[a * a + a + b] = {sg * (R)mv.re, sg * (R)mv.im};
+      if (b > 0) sM[a * a + a - b] = {(R)mv.re, -(R)mv.im};
     }
-    const double scale = mass ? mi0 * mj0 : 1.0;
     __syncthreads();
     const char *mb = reinterpret_cast<const char *>(sM);
     const char *sb = reinterpret_cast<const char *>(sS);
-    double ar = 0, ai = 0;
+    R ar = 0, ai = 0;
     int slot = myslot0;
 #pragma unroll
     for (int it = 0; it < IPL; it++) {
       const uint32_t d = desc[it];
-      const cplx mv = *reinterpret_cast<const cplx *>(mb + (d & 0xFFF));
-      const cplx sv = *reinterpret_cast<const cplx *>(sb + ((d >> 12) & 0xFFF));
+      const C mv = *reinterpret_cast<const C *>(mb + (d & 0xFFF));
+      const C sv = *reinterpret_cast<const C *>(sb + ((d >> 12) & 0xFFF));
       ar = fma(mv.re, sv.re, ar);
       ar = fma(-mv.im, sv.im, ar);
       ai = fma(mv.re, sv.im, ai);
       ai = fma(mv.im, sv.re, ai);
       if (d & (1u << 24)) {
+        const double f = F32 ? sU[d >> 25] : scale;
         cplx t = seg[slot];
-        t.re += scale * ar;
-        t.im += scale * ai;
+        t.re += f * (double)ar;
+        t.im += f * (double)ai;
         seg[slot] = t;
         slot++;
         ar = ai = 0;
@@ -261,8 +291,10 @@ __global__ void __launch_bounds__(64)
 template <int P>
 static void m2l_impl(Handle &h) {
   using S = M2LShape<P>;
-  if (h.m2l_sched_P != P) {
-    const M2LSchedule sc = build_schedule(P, S::IPL);
+  const int esz = h.cfg.precision == 1 ? 8 : 16;  // bytes of one table entry
+  const int key = P * 100 + esz;
+  if (h.m2l_sched_P != key) {
+    const M2LSchedule sc = build_schedule(P, S::IPL, esz);
     std::vector<uint32_t> blob(sc.desc);
     for (int v : sc.slot0) blob.push_back((uint32_t)v);
     for (int v : sc.outslot) blob.push_back((uint32_t)v);
@@ -270,13 +302,17 @@ static void m2l_impl(Handle &h) {
     EXA_HIP(hipMemcpyAsync(h.m2l_sched.p, blob.data(), blob.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                            h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
-    h.m2l_sched_P = P;
+    h.m2l_sched_P = key;
   }
   const uint32_t *desc = h.m2l_sched.p;
   const int *slot0 = reinterpret_cast<const int *>(desc + 64 * S::IPL);
   const int *outslot = slot0 + 64;
-  hipLaunchKernelGGL(k_m2l_f64<P>, dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p, h.m2l_src.p,
-                     h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+  if (esz == 8)
+    hipLaunchKernelGGL((k_m2l<P, float>), dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p,
+                       h.m2l_src.p, h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+  else
+    hipLaunchKernelGGL((k_m2l<P, double>), dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p,
+                       h.m2l_src.p, h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
   EXA_HIP(hipGetLastError());
 }
 

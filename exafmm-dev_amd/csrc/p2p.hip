@@ -106,6 +106,131 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Mixed precision (cfg.precision = 1): FP32 pair arithmetic on positions
+// taken relative to the target leaf's centre in FP64 (the source's image
+// shift folded in) and split into FP32 hi + lo parts, so the FP32 values are
+// cell-sized and x_i - x_j = (hi_i - hi_j) + (lo_i - lo_j) keeps ~FP32
+// relative accuracy even for bodies much closer to each other than the cell
+// size (clustered distributions; a plain FP32 difference there would carry
+// an error of one FP32 ulp of the cell size).  Each lane sums runs of kSeg
+// sources in FP32 and adds each run's sum into a compensated (Kahan)
+// accumulator: the per-target compensated summation of the reference's
+// kahan<T> (kahan.h:35-41), applied per run instead of per term.  The
+// compensated value is read out as s - c.
+constexpr int kP2PStageF = 256;
+constexpr int kSeg = 8;  // FP32 pair terms summed before each compensated add
+
+struct Kahan32 {
+  float s = 0, c = 0;
+  __device__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ double value() const { return (double)s - (double)c; }
+};
+
+__global__ void __launch_bounds__(64)
+    k_p2p_mixed(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
+                const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
+                const double4 *__restrict__ cX, const double4 *__restrict__ body, double4 *__restrict__ trg,
+                double cycle) {
+  __shared__ float4 sb[kP2PStageF];   // hi parts and charge
+  __shared__ float4 sbl[kP2PStageF];  // lo parts
+  __shared__ double4 sacc[64];
+  const int c = tgt_list[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int tb0 = cBody[c], tn = cNBody[c];
+  const double4 Xc = cX[c];
+  const int T = tn <= 16 ? 16 : (tn <= 32 ? 32 : 64);
+  const int G = 64 / T;
+  const int g = lane / T, ti = lane % T;
+  const int e0 = off[c], e1 = off[c + 1];
+  for (int tbase = 0; tbase < tn; tbase += T) {
+    const bool active = tbase + ti < tn;
+    const double4 xd = active ? body[tb0 + tbase + ti] : Xc;
+    const double rx = xd.x - Xc.x, ry = xd.y - Xc.y, rz = xd.z - Xc.z;
+    const float xi = (float)rx, yi = (float)ry, zi = (float)rz;
+    const float xil = (float)(rx - xi), yil = (float)(ry - yi), zil = (float)(rz - zi);
+    Kahan32 kp, kx, ky, kz;
+    int e = e0, sbase = 0;
+    while (e < e1) {
+      int cnt = 0;
+      __syncthreads();
+      while (e < e1 && cnt < kP2PStageF) {
+        const int s = src[e];
+        const int sb0 = cBody[s], sn = cNBody[s];
+        double px = 0, py = 0, pz = 0;
+        image_shift(img[e], cycle, px, py, pz);
+        const int take = min(sn - sbase, kP2PStageF - cnt);
+        for (int k = lane; k < take; k += 64) {
+          const double4 v = body[sb0 + sbase + k];
+          const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
+          const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+          sb[cnt + k] = make_float4(hx, hy, hz, (float)v.w);
+          sbl[cnt + k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
+        }
+        cnt += take;
+        sbase += take;
+        if (sbase == sn) {
+          e++;
+          sbase = 0;
+        }
+      }
+      // Pad the tile to whole segments with zero charges (they add exactly 0).
+      const int cntp = (cnt + kSeg * G - 1) / (kSeg * G) * (kSeg * G);
+      for (int k = cnt + lane; k < cntp; k += 64) {
+        sb[k] = make_float4(0, 0, 0, 0);
+        sbl[k] = make_float4(0, 0, 0, 0);
+      }
+      __syncthreads();
+      for (int k0 = g; k0 < cntp; k0 += kSeg * G) {
+        float pot = 0, ax = 0, ay = 0, az = 0;
+#pragma unroll
+        for (int u = 0; u < kSeg; u++) {
+          const int k = k0 + u * G;
+          const float4 xj = sb[k];
+          const float4 xl = sbl[k];
+          const float dx = (xi - xj.x) + (xil - xl.x), dy = (yi - xj.y) + (yil - xl.y),
+                      dz = (zi - xj.z) + (zil - xl.z);
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          const float invr = r2 != 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+          const float qinvr = xj.w * invr;
+          const float qinvr3 = qinvr * invr * invr;
+          pot += qinvr;
+          ax += dx * qinvr3;
+          ay += dy * qinvr3;
+          az += dz * qinvr3;
+        }
+        kp.add(pot);
+        kx.add(ax);
+        ky.add(ay);
+        kz.add(az);
+      }
+    }
+    __syncthreads();
+    sacc[lane] = make_double4(kp.value(), kx.value(), ky.value(), kz.value());
+    __syncthreads();
+    if (g == 0 && active) {
+      double4 s = sacc[ti];
+      for (int gg = 1; gg < G; gg++) {
+        const double4 o = sacc[gg * T + ti];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tbase + ti];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tbase + ti] = tv;
+    }
+  }
+}
+
 // Target-source body pairs of the P2P list (for the interactions/s metric).
 __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, const int *__restrict__ off,
                                    const int *__restrict__ src, const int *__restrict__ cNBody,
@@ -121,8 +246,12 @@ __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, co
 void run_p2p(Handle &h, hipStream_t st) {
   EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), st));
   if (h.np2p_tgt == 0) return;
-  hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
-                     h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+  if (h.cfg.precision == 1)
+    hipLaunchKernelGGL(k_p2p_mixed, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
+                       h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+  else
+    hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
+                       h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   EXA_HIP(hipGetLastError());
 }
 
