@@ -376,6 +376,32 @@ int exafmm_hip_direct_arrays(exafmm_hip_handle *H, int nt, const double *t_xyzq,
   });
 }
 
+int exafmm_hip_get_dipole(exafmm_hip_handle *H, const double *X0, double *dipole) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_tree && dipole, "get_dipole: no tree or null output");
+    const double zero[3] = {0, 0, 0};
+    run_get_dipole(h, X0 ? X0 : zero, dipole);
+  });
+}
+
+int exafmm_hip_dipole_correction(exafmm_hip_handle *H, const double *dipole, int numBodies, double cycle) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_trav && dipole && numBodies > 0 && cycle > 0, "dipole_correction: bad arguments or no results");
+    run_dipole_correction(h, dipole, numBodies, cycle);
+    EXA_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+int exafmm_hip_ewald(exafmm_hip_handle *H, int ksize, double alpha, double sigma, double cutoff, double cycle,
+                     int parts, double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(h.have_tree, "ewald: tree not built");
+    ensure(trg && ksize >= 0 && alpha > 0 && sigma > 0 && cutoff > 0 && cycle > 0 && parts >= 0 && parts <= 7,
+           "ewald: bad arguments");
+    run_ewald(h, ksize, alpha, sigma, cutoff, cycle, parts, trg);
+  });
+}
+
 int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
   return guarded(H, [&](Handle &h) {
     ensure(s != nullptr, "get_stats: null");
@@ -396,7 +422,3 @@ int exafmm_hip_synchronize(exafmm_hip_handle *H) {
 }
 
 }  // extern "C"
-
-namespace exafmm {
-void run_periodic(Handle &) { throw Error(EXAFMM_HIP_EUNSUP, "periodic images not available yet"); }
-}  // namespace exafmm

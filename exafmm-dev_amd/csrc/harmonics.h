@@ -137,4 +137,39 @@ __device__ inline void singular_nm(int n, int m, const Geom &g, double &re, doub
   im = p * s;
 }
 
+__device__ inline void coef_nm(int t, int &n, int &m) {
+  n = 0;
+  while (tri(n + 1) <= t) n++;
+  m = t - tri(n);
+}
+
+// One output coefficient (j, k) of M2M (LaplaceSphericalCPU.cxx:120-147):
+// the child multipole sM (packed, m >= 0) shifted by the regular harmonics
+// sY (packed, m >= 0) of X_parent - X_child; adds into (lr, li).
+template <typename C>
+__device__ inline void m2m_lane(int j, int k, const C *sM, const C *sY, double &lr, double &li) {
+  for (int n = 0; n <= j; n++) {
+    // m from max(-n, -j+k+n) to min(k-1, n): source (j-n, k-m), Y_n^{-m}
+    const int mlo = max(-n, -j + k + n), mhi = min(k - 1, n);
+    for (int m = mlo; m <= mhi; m++) {
+      const C s = sM[tri(j - n) + k - m];
+      C y = sY[tri(n) + (m >= 0 ? m : -m)];
+      if (m >= 0) y.im = -y.im;  // Y_n^{-m} = conj(Y_n^m)
+      const double f = ((m < 0 && (m & 1)) ? -1.0 : 1.0) * ((n & 1) ? -1.0 : 1.0);
+      lr += f * (s.re * y.re - s.im * y.im);
+      li += f * (s.re * y.im + s.im * y.re);
+    }
+    // m from k to min(n, j+k-n): conj(source (j-n, m-k)), Y_n^{-m}
+    const int mhi2 = min(n, j + k - n);
+    for (int m = k; m <= mhi2; m++) {
+      const C s = sM[tri(j - n) - k + m];
+      const C y = sY[tri(n) + m];
+      // conj(s) * conj(y) = conj(s * y)
+      const double f = ((k + n + m) & 1) ? -1.0 : 1.0;
+      lr += f * (s.re * y.re - s.im * y.im);
+      li -= f * (s.re * y.im + s.im * y.re);
+    }
+  }
+}
+
 }  // namespace exafmm

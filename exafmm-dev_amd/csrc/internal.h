@@ -106,6 +106,14 @@ struct Handle {
   DBuf<int> at_start, at_count, at_parent, at_leaf, at_level, at_first, at_nchild, at_astart, at_acount, at_agid,
       at_idx2, at_nnode, at_cellid, at_cursor, at_nl;
 
+  // Periodic far field: the periodic parent's multipole and 26 partial
+  // local expansions per sublevel.
+  DBuf<cplx> per_M, per_part;
+
+  // Ewald summation: waves (kx, ky, kz as doubles, re, im) and results.
+  DBuf<double> ew_waves;
+  DBuf<double4> ew_trg;
+
   // M2L lane schedule (a pure function of P).
   DBuf<uint32_t> m2l_sched;
   int m2l_sched_P = -1;
@@ -138,6 +146,10 @@ void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
 void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg);
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
+void run_get_dipole(Handle &h, const double *X0, double *out);
+void run_dipole_correction(Handle &h, const double *dipole, int numBodies, double cycle);
+void run_ewald(Handle &h, int ksize, double alpha, double sigma, double cutoff, double cycle, int parts,
+               double *trg);
 void run_periodic(Handle &h);
 int64_t count_p2p_body_pairs(Handle &h);
 

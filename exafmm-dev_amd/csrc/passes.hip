@@ -12,12 +12,6 @@
 
 namespace exafmm {
 
-__device__ inline void coef_nm(int t, int &n, int &m) {
-  n = 0;
-  while (tri(n + 1) <= t) n++;
-  m = t - tri(n);
-}
-
 // |a - b| with the reference's rounding: norm() sums x^2, y^2, z^2 left to
 // right without contraction (vec.h:163-167), then std::sqrt.
 __device__ inline double exact_dist(double ax, double ay, double az, double bx, double by, double bz) {
@@ -106,30 +100,8 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int o = 0; o < NO; o++) {
         if (lane + 64 * o >= NT) continue;
-        const int j = tn[o], k = tm[o];
         double lr = 0, li = 0;
-        for (int n = 0; n <= j; n++) {
-          // m from max(-n, -j+k+n) to min(k-1, n): source (j-n, k-m), Y_n^{-m}
-          const int mlo = max(-n, -j + k + n), mhi = min(k - 1, n);
-          for (int m = mlo; m <= mhi; m++) {
-            const cplx s = sM[tri(j - n) + k - m];
-            cplx y = sY[tri(n) + (m >= 0 ? m : -m)];
-            if (m >= 0) y.im = -y.im;  // Y_n^{-m} = conj(Y_n^m)
-            const double f = ((m < 0 && (m & 1)) ? -1.0 : 1.0) * ((n & 1) ? -1.0 : 1.0);
-            lr += f * (s.re * y.re - s.im * y.im);
-            li += f * (s.re * y.im + s.im * y.re);
-          }
-          // m from k to min(n, j+k-n): conj(source (j-n, m-k)), Y_n^{-m}
-          const int mhi2 = min(n, j + k - n);
-          for (int m = k; m <= mhi2; m++) {
-            const cplx s = sM[tri(j - n) - k + m];
-            const cplx y = sY[tri(n) + m];
-            // conj(s) * conj(y) = conj(s * y)
-            const double f = ((k + n + m) & 1) ? -1.0 : 1.0;
-            lr += f * (s.re * y.re - s.im * y.im);
-            li -= f * (s.re * y.im + s.im * y.re);
-          }
-        }
+        m2m_lane(tn[o], tm[o], sM, sY, lr, li);
         acc_re[o] += lr;
         acc_im[o] += li;
       }

@@ -23,6 +23,7 @@ EXPORTS = (
     "exafmm_hip_traverse", "exafmm_hip_downward_pass", "exafmm_hip_evaluate", "exafmm_hip_get_cells",
     "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
     "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
+    "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald",
 )
 
 
@@ -92,6 +93,9 @@ def load():
     L.exafmm_hip_direct_arrays.argtypes = [vp, ip, vp, ip, vp, vp]
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
+    L.exafmm_hip_get_dipole.argtypes = [vp, vp, vp]
+    L.exafmm_hip_dipole_correction.argtypes = [vp, vp, ip, dp]
+    L.exafmm_hip_ewald.argtypes = [vp, ip, dp, dp, dp, dp, ip, vp]
     for f in EXPORTS:
         if f != "exafmm_hip_last_error":
             getattr(L, f).restype = ctypes.c_int
@@ -230,6 +234,28 @@ class FMM:
         out = np.zeros((t.shape[0], 4))
         self._check(self.lib.exafmm_hip_direct_arrays(self.h, t.shape[0], _ptr(t), s.shape[0], _ptr(s), _ptr(out)),
                     "direct_arrays")
+        return out
+
+    def dipole(self, X0=(0.0, 0.0, 0.0)):
+        """UpDownPass::getDipole: sum of (X - X0) q over the bodies."""
+        x0 = np.ascontiguousarray(X0, np.float64)
+        out = np.zeros(3)
+        self._check(self.lib.exafmm_hip_get_dipole(self.h, _ptr(x0), _ptr(out)), "get_dipole")
+        return out
+
+    def dipole_correction(self, dipole, num_bodies, cycle):
+        """UpDownPass::dipoleCorrection on the device-resident results."""
+        d = np.ascontiguousarray(dipole, np.float64)
+        self._check(self.lib.exafmm_hip_dipole_correction(self.h, _ptr(d), int(num_bodies), float(cycle)),
+                    "dipole_correction")
+
+    def ewald(self, ksize, alpha, sigma, cutoff, cycle, parts=7, trg=None):
+        """Ewald summation (wave 1, real 2, self 4) added to trg (n x 4,
+        caller order; zeros when None), which is returned."""
+        n = self.n
+        out = np.zeros((n, 4)) if trg is None else np.ascontiguousarray(trg, np.float64)
+        self._check(self.lib.exafmm_hip_ewald(self.h, int(ksize), float(alpha), float(sigma), float(cutoff),
+                                            float(cycle), int(parts), _ptr(out)), "ewald")
         return out
 
     def stats(self):

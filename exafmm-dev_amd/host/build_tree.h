@@ -26,6 +26,9 @@ class BuildTree {
     logger::startTimer("Grow tree");
     exafmm_host::Context &ctx = exafmm_host::context();
     ctx.cfg.ncrit = ncrit;
+#ifdef EXAFMM_HOST_BUILDER
+    ctx.cfg.builder = EXAFMM_HOST_BUILDER;  // picked by build_tree_omp.h / build_tree_tbb.h
+#endif
     ctx.apply();
     Cells cells;
     if (bodies.empty()) {

@@ -51,6 +51,25 @@ class UpDownPass {
     logger::stopTimer("Downward pass");
   }
 
+  // getDipole (include/up_down_pass.h:151-157) over caller-owned host
+  // bodies (any body set, e.g. sampled targets); device-resident results
+  // use exafmm_hip_get_dipole.
+  vec3 getDipole(Bodies &bodies, vec3 X0) {
+    vec3 dipole = 0;
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++) dipole += (B->X - X0) * B->SRC;
+    return dipole;
+  }
+
+  // dipoleCorrection (include/up_down_pass.h:160-168) on host Body::TRG;
+  // exafmm_hip_dipole_correction applies the same to device-resident results.
+  void dipoleCorrection(Bodies &bodies, vec3 dipole, int numBodies, real_t cycle) {
+    const real_t coef = 4 * M_PI / (3 * cycle * cycle * cycle);
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++) {
+      B->TRG[0] -= coef * norm(dipole) / numBodies / B->SRC;
+      for (int d = 0; d < 3; d++) B->TRG[d + 1] -= coef * dipole[d];
+    }
+  }
+
   // Copy the device expansions into Cell::M and Cell::L (host inspection).
   void fetchExpansions(Cells &cells) {
     exafmm_host::Context &ctx = exafmm_host::context();

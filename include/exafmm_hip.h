@@ -130,6 +130,21 @@ int exafmm_hip_direct(exafmm_hip_handle *h, int nt, const int *target_ibody, dou
 int exafmm_hip_direct_arrays(exafmm_hip_handle *h, int nt, const double *t_xyzq, int ns, const double *s_xyzq,
                              double *trg);
 
+/* UpDownPass::getDipole (include/up_down_pass.h:151-157): dipole of the
+ * bodies about X0 (NULL = origin), sum of (X - X0) q.  Needs the tree. */
+int exafmm_hip_get_dipole(exafmm_hip_handle *h, const double *X0, double *dipole);
+/* UpDownPass::dipoleCorrection (include/up_down_pass.h:160-168) applied to
+ * the results on the device: pot -= coef |D|^2 / numBodies / q,
+ * grad -= coef D, coef = 4 pi / (3 cycle^3). */
+int exafmm_hip_dipole_correction(exafmm_hip_handle *h, const double *dipole, int numBodies, double cycle);
+/* Ewald summation (include/ewald.h:25-181): parts = 1 wave part
+ * (Ewald::wavePart), 2 real part over the tree (Ewald::realPart), 4 self
+ * term (Ewald::selfTerm), or any sum of them.  The selected parts are ADDED
+ * to trg (n x 4, caller body order), as the reference adds them to
+ * Body::TRG.  Needs the tree (the real part walks it). */
+int exafmm_hip_ewald(exafmm_hip_handle *h, int ksize, double alpha, double sigma, double cutoff, double cycle,
+                     int parts, double *trg);
+
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 
 /* Wait for all work queued on the handle's stream. */

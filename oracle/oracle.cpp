@@ -927,6 +927,140 @@ struct orc_ctx {
   }
 
   // Downward pass, up_down_pass.h:87-111,135-148 (pre-order).
+  // UpDownPass::getDipole, up_down_pass.h:151-157 (bodies in tree order).
+  void get_dipole(const double *X0, double *dp) const {
+    double d[3] = {0, 0, 0};
+    for (int b = 0; b < n; b++)
+      for (int k = 0; k < 3; k++) d[k] += (X[3 * (size_t)b + k] - X0[k]) * Q[b];
+    for (int k = 0; k < 3; k++) dp[k] = d[k];
+  }
+
+  // UpDownPass::dipoleCorrection, up_down_pass.h:160-168.
+  void dipole_correction(const double *dp, int numBodies, double cycle) {
+    const double coef = 4 * M_PI / (3 * cycle * cycle * cycle);
+    const double nd = norm3(dp);
+    for (int b = 0; b < n; b++) {
+      trg[4 * (size_t)b] -= coef * nd / numBodies / Q[b];
+      for (int k = 0; k < 3; k++) trg[4 * (size_t)b + 1 + k] -= coef * dp[k];
+    }
+  }
+
+  // Ewald summation, ewald.h:25-181, on the context's bodies and tree.  The
+  // source tree of examples/ewald.cxx:101-103 is rebuilt from the tree-ordered
+  // bodies with their own bounds, which reproduces the same tree and order,
+  // so the target tree serves as the source tree.  out: 4 per body, tree order.
+  void ewald(int ksize, double alpha, double sigma, double cutoff, double cycle, double *out) const {
+    std::fill(out, out + 4 * (size_t)n, 0.0);
+    // initWaves, ewald.h:64-90.
+    std::vector<int> K;
+    const int kmaxsq = ksize * ksize, kmax = ksize;
+    for (int l = 0; l <= kmax; l++) {
+      const int mmin = l == 0 ? 0 : -kmax;
+      for (int m = mmin; m <= kmax; m++) {
+        const int nmin = (l == 0 && m == 0) ? 1 : -kmax;
+        for (int nn = nmin; nn <= kmax; nn++) {
+          const double ksq = l * l + m * m + nn * nn;
+          if (ksq <= kmaxsq) {
+            K.push_back(l);
+            K.push_back(m);
+            K.push_back(nn);
+          }
+        }
+      }
+    }
+    const int nw = (int)K.size() / 3;
+    std::vector<double> WR(nw), WI(nw);
+    const double scale = 2 * M_PI / cycle;
+    // dft, ewald.h:25-40.
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int w = 0; w < nw; w++) {
+      double re = 0, im = 0;
+      for (int b = 0; b < n; b++) {
+        double th = 0;
+        for (int d = 0; d < 3; d++) th += K[3 * w + d] * X[3 * (size_t)b + d] * scale;
+        re += Q[b] * std::cos(th);
+        im += Q[b] * std::sin(th);
+      }
+      WR[w] = re;
+      WI[w] = im;
+    }
+    // wavePart factors, ewald.h:166-181.
+    const double coef = .5 / M_PI / M_PI / sigma / cycle;
+    const double coef2 = scale * scale / (4 * alpha * alpha);
+    for (int w = 0; w < nw; w++) {
+      const double kv[3] = {(double)K[3 * w], (double)K[3 * w + 1], (double)K[3 * w + 2]};
+      const double K2 = norm3(kv);
+      const double factor = coef * std::exp(-K2 * coef2) / K2;
+      WR[w] *= factor;
+      WI[w] *= factor;
+    }
+    // idft, ewald.h:43-61.
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int b = 0; b < n; b++) {
+      double t[4] = {0, 0, 0, 0};
+      for (int w = 0; w < nw; w++) {
+        double th = 0;
+        for (int d = 0; d < 3; d++) th += K[3 * w + d] * X[3 * (size_t)b + d] * scale;
+        const double dtmp = WR[w] * std::sin(th) - WI[w] * std::cos(th);
+        t[0] += WR[w] * std::cos(th) + WI[w] * std::sin(th);
+        for (int d = 0; d < 3; d++) t[d + 1] -= dtmp * K[3 * w + d];
+      }
+      for (int d = 0; d < 3; d++) t[d + 1] *= scale;
+      for (int d = 0; d < 4; d++) out[4 * (size_t)b + d] += t[d];
+    }
+    // realPart, ewald.h:145-156: every target leaf walks the source tree.
+    std::vector<int> leaves;
+    for (int c = 0; c < (int)cells.size(); c++)
+      if (cells[c].nchild == 0) leaves.push_back(c);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int li = 0; li < (int)leaves.size(); li++) ewald_neighbor(leaves[li], 0, alpha, cutoff, cycle, out);
+    // selfTerm, ewald.h:159-163.
+    for (int b = 0; b < n; b++) out[4 * (size_t)b] -= M_2_SQRTPI * Q[b] * alpha;
+  }
+
+  // Ewald::Neighbor, ewald.h:118-137.
+  void ewald_neighbor(int ci, int cj, double alpha, double cutoff, double cycle, double *out) const {
+    const OCell &Ci = cells[ci], &Cj = cells[cj];
+    double dX[3], Xp[3];
+    for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - Cj.X[d];
+    for (int d = 0; d < 3; d++) {  // wrap, vec.h:193-205
+      if (dX[d] < -cycle / 2) dX[d] += cycle;
+      if (dX[d] > cycle / 2) dX[d] -= cycle;
+    }
+    for (int d = 0; d < 3; d++) Xp[d] = Ci.X[d] - Cj.X[d] - dX[d];
+    const double R = std::sqrt(norm3(dX));
+    if (R < 3 * cutoff) {
+      if (Cj.nchild == 0) ewald_p2p(ci, cj, Xp, alpha, cutoff, out);
+      for (int c = Cj.ichild; c < Cj.ichild + Cj.nchild; c++) ewald_neighbor(ci, c, alpha, cutoff, cycle, out);
+    }
+  }
+
+  // Ewald::P2P (real-space part), ewald.h:93-115.
+  void ewald_p2p(int ci, int cj, const double *Xp, double alpha, double cutoff, double *out) const {
+    const OCell &Ci = cells[ci], &Cj = cells[cj];
+    for (int i = Ci.ibody; i < Ci.ibody + Ci.nbody; i++) {
+      double *t = out + 4 * (size_t)i;
+      for (int j = Cj.ibody; j < Cj.ibody + Cj.nbody; j++) {
+        double dX[3];
+        for (int d = 0; d < 3; d++) dX[d] = X[3 * (size_t)i + d] - X[3 * (size_t)j + d] - Xp[d];
+        const double R2 = norm3(dX);
+        if (0 < R2 && R2 < cutoff * cutoff) {
+          const double R2s = R2 * alpha * alpha;
+          const double Rs = std::sqrt(R2s);
+          const double invRs = 1 / Rs;
+          const double invR2s = invRs * invRs;
+          const double invR3s = invR2s * invRs;
+          double dtmp = Q[j] * (M_2_SQRTPI * std::exp(-R2s) * invR2s + std::erfc(Rs) * invR3s);
+          dtmp *= alpha * alpha * alpha;
+          t[0] += Q[j] * std::erfc(Rs) * invRs * alpha;
+          t[1] -= dX[0] * dtmp;
+          t[2] -= dX[1] * dtmp;
+          t[3] -= dX[2] * dtmp;
+        }
+      }
+    }
+  }
+
   void downward() {
     if (cells.empty()) return;
     if (cells[0].nchild == 0) L2P(0);
@@ -1029,6 +1163,18 @@ void orc_get_bodies(const orc_ctx *c, double *xyzq, int *ibody, uint64_t *key, d
     if (trg)
       for (int d = 0; d < 4; d++) trg[4 * (size_t)b + d] = c->trg[4 * (size_t)b + d];
   }
+}
+
+void orc_get_dipole(const orc_ctx *c, const double *X0, double *dipole) { c->get_dipole(X0, dipole); }
+
+void orc_dipole_correction(orc_ctx *c, const double *dipole, int numBodies, double cycle) {
+  c->dipole_correction(dipole, numBodies, cycle);
+}
+
+int orc_ewald(const orc_ctx *c, int ksize, double alpha, double sigma, double cutoff, double cycle, double *out) {
+  if (c->cells.empty() || ksize < 0 || !(alpha > 0) || !(sigma > 0) || !(cycle > 0)) return -1;
+  c->ewald(ksize, alpha, sigma, cutoff, cycle, out);
+  return 0;
 }
 
 long orc_num_p2p(const orc_ctx *c) { return (long)c->p2p_i.size(); }

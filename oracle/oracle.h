@@ -56,6 +56,14 @@ void orc_get_cells(const orc_ctx *c, int *ints5, uint64_t *icell, double *X, dou
                    double *M, double *L);
 /* Bodies in tree order: xyzq (4/body), original index, key, trg (4/body). */
 void orc_get_bodies(const orc_ctx *c, double *xyzq, int *ibody, uint64_t *key, double *trg);
+/* UpDownPass::getDipole (up_down_pass.h:151-157) over the bodies. */
+void orc_get_dipole(const orc_ctx *c, const double *X0, double *dipole);
+/* UpDownPass::dipoleCorrection (up_down_pass.h:160-168) applied to trg. */
+void orc_dipole_correction(orc_ctx *c, const double *dipole, int numBodies, double cycle);
+/* Ewald summation (ewald.h:25-181: wave part, real part over the tree,
+ * self term) onto every body; out = 4 per body in tree order.  Needs the
+ * tree (stage 2). */
+int orc_ewald(const orc_ctx *c, int ksize, double alpha, double sigma, double cutoff, double cycle, double *out);
 long orc_num_p2p(const orc_ctx *c);
 long orc_num_m2l(const orc_ctx *c);
 void orc_get_lists(const orc_ctx *c, int *p2p_i, int *p2p_j, int *m2l_i, int *m2l_j);

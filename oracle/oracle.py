@@ -58,6 +58,10 @@ def lib():
         L.orc_direct.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int]
         L.orc_reference_dataset.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_drand48_seq.argtypes = [ctypes.c_long, ctypes.c_int, dp]
+        L.orc_get_dipole.argtypes = [vp, dp, dp]
+        L.orc_dipole_correction.argtypes = [vp, dp, ctypes.c_int, ctypes.c_double]
+        L.orc_ewald.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                dp]
         _lib = L
     return _lib
 
@@ -123,6 +127,27 @@ class Oracle:
         trg = np.zeros((n, 4))
         lib().orc_get_bodies(self.h, _vp(xyzq), _vp(ib), _vp(key), _vp(trg))
         return dict(xyzq=xyzq, ibody=ib, key=key, trg=trg)
+
+    def dipole(self, X0=(0.0, 0.0, 0.0)):
+        x0 = np.ascontiguousarray(X0, np.float64)
+        out = np.zeros(3)
+        lib().orc_get_dipole(self.h, _dp(x0), _dp(out))
+        return out
+
+    def dipole_correction(self, dipole, num_bodies, cycle):
+        d = np.ascontiguousarray(dipole, np.float64)
+        lib().orc_dipole_correction(self.h, _dp(d), int(num_bodies), float(cycle))
+
+    def ewald(self, ksize, alpha, sigma, cutoff, cycle):
+        """Ewald potential and gradient of every body, in caller order."""
+        n = lib().orc_num_bodies(self.h)
+        out = np.zeros((n, 4))
+        rc = lib().orc_ewald(self.h, ksize, alpha, sigma, cutoff, cycle, _dp(out))
+        if rc != 0:
+            raise ValueError("orc_ewald: invalid arguments or no tree")
+        res = np.empty_like(out)
+        res[self.bodies()["ibody"]] = out
+        return res
 
     def lists(self):
         n1 = lib().orc_num_p2p(self.h)

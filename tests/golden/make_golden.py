@@ -36,6 +36,8 @@ CASES = [
     ("cube4k_a_p12", "harness_a_p12", ["gen", "c", "4096"], [0.4, 32, 1000, 1, 0, 0, 0], "tree"),
     ("octant8k_a_p6", "harness_a_p6", ["gen", "o", "8192"], [0.4, 64, 1000, 1, 0, 0, 0], "tree"),
     ("cube2k_u_p6_periodic", "harness_u_p6", ["gen", "c", "2048"], [0.4, 64, 1000, 1, 0, 0, 2], "tree"),
+    ("mass_cube2k_u_p8_img3", "harness_u_p8_mass", ["gen", "c", "2048"], [0.4, 64, 1000, 1, 0, 0, 3], "tree"),
+    ("cube4k_a_p10_img3", "harness_a_p10", ["gen", "c", "4096"], [0.4, 64, 1000, 1, 0, 0, 3], "tree"),
     # Inputs of the BASELINE-style generator (U[-1,1]^3, zero-mean charges / N).
     ("unit16k_u_p10", "harness_u_p10", ["file"], [0.4, 64, 1000, 1, 0, 0, 0], "tree"),
     ("small100_u_p10", "harness_u_p10", ["gen", "c", "100"], [0.4, 64, 1000, 1, 0, 0, 0], "full"),
@@ -47,6 +49,53 @@ CASES = [
     ("cube16k_a_p10_mut", "harness_a_p10", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
     ("mass_plummer16k_a_p10_mut", "harness_a_p10_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 1, 0], "tree"),
 ]
+
+
+# Periodic cases for oracle/ref_ewald.cxx: name, binary, input, options
+# (theta ncrit nspawn images ksize alpha sigma cutoff cycle).
+CYCLE = 2 * np.pi
+EWALD_CASES = [
+    ("lattice12_p10_img3_ewald", "ref_ewald_p10", ("lattice", 12, 1),
+     [0.35, 64, 1000, 3, 11, 10 / CYCLE, 0.25 / np.pi, CYCLE / 2, CYCLE]),
+    ("lattice12_p6_img2_ewald", "ref_ewald_p6", ("lattice", 12, 2),
+     [0.35, 64, 1000, 2, 11, 10 / CYCLE, 0.25 / np.pi, 10.0, CYCLE]),
+    ("cube2k_p10_img4_ewald", "ref_ewald_p10", ("cube", 2048, 3),
+     [0.4, 32, 1000, 4, 11, 10 / CYCLE, 0.25 / np.pi, CYCLE / 2, CYCLE]),
+]
+
+
+def jittered_lattice(m, seed):
+    """m^3 bodies on a cubic lattice in [0, 2 pi)^3, each moved by a uniform
+    jitter of +-0.25 spacing, with alternating +-1 charges (neutral for even m)."""
+    h = CYCLE / m
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    x = (g + 0.5) * h + rng.uniform(-0.25, 0.25, g.shape) * h
+    q = np.where(g.sum(1) % 2 == 0, 1.0, -1.0)
+    return np.concatenate([x, q[:, None]], axis=1)
+
+
+def periodic_cube(n, seed):
+    """n bodies uniform in [0, 2 pi)^3 with zero-mean charges."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, CYCLE, (n, 3))
+    q = rng.uniform(-0.5, 0.5, n)
+    q -= q.mean()
+    return np.concatenate([x, q[:, None]], axis=1)
+
+
+def read_ewald_dump(path):
+    b = open(path, "rb").read()
+    assert b[:4] == b"EXEW"
+    n = int(np.frombuffer(b, np.int32, 1, 4)[0])
+    off = 8
+    dip = np.frombuffer(b, np.float64, 3, off).copy()
+    off += 24
+    out = dict(dipole=dip)
+    for k in ("fmm_raw", "fmm", "ewald"):
+        out[k] = np.frombuffer(b, np.float64, 4 * n, off).reshape(n, 4).copy()
+        off += 32 * n
+    return out
 
 
 def unit_cube(n, seed):
@@ -101,6 +150,21 @@ def main():
                 d["M"] = c["M"]
                 d["L"] = c["L"]
             np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+        for name, binary, (kind, size, seed), opts in EWALD_CASES:
+            x = jittered_lattice(size, seed) if kind == "lattice" else periodic_cube(size, seed)
+            inp = os.path.join(td, name + ".in")
+            out = os.path.join(td, name + ".bin")
+            with open(inp, "wb") as f:
+                np.array([x.shape[0]], np.int32).tofile(f)
+                x.tofile(f)
+            args = [os.path.join(REF, binary), inp, out] + [repr(float(v)) if isinstance(v, float) else str(v)
+                                                            for v in opts]
+            print(" ".join(args[1:]), flush=True)
+            subprocess.run(args, check=True)
+            r = read_ewald_dump(out)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), input=x, theta=opts[0], ncrit=opts[1],
+                                nspawn=opts[2], images=opts[3], ksize=opts[4], alpha=opts[5], sigma=opts[6],
+                                cutoff=opts[7], cycle=opts[8], P=int(binary.rsplit("_p", 1)[1]), **r)
     # Generator fixtures: drand48 and the first bodies of every distribution.
     g = dict(drand48_seed0=O.drand48_seq(0, 16), drand48_seed42=O.drand48_seq(42, 16))
     for dist in "cspol":

@@ -15,8 +15,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "exafmm-dev_amd", "host")
-HEADERS = ["args.h", "bound_box.h", "build_tree.h", "dataset.h", "kernel.h", "logger.h", "thread.h", "traversal.h",
-           "types.h", "up_down_pass.h", "verify.h"]
+HEADERS = ["args.h", "bound_box.h", "build_tree_omp.h", "build_tree.h", "dataset.h", "ewald.h", "kernel.h", "logger.h",
+           "thread.h", "traversal.h", "types.h", "up_down_pass.h", "verify.h"]
 
 
 def test_host_headers_compile_and_link(tmp_path):
@@ -25,6 +25,8 @@ def test_host_headers_compile_and_link(tmp_path):
                    "int main() { Args a; Dataset d; Bodies b = d.initBodies(100, \"cube\"); Verify v;\n"
                    "  BoundBox bb(a.nspawn); BuildTree bt(a.ncrit, a.nspawn); UpDownPass up(a.theta, 1, 0);\n"
                    "  Traversal tr(a.nspawn, a.images, 0); (void)bb; (void)bt; (void)up; (void)tr; (void)v;\n"
+                   "  Ewald ew(11, 1.0, 0.1, 3.0, 6.28); (void)ew; vec3 dp = up.getDipole(b, 0);\n"
+                   "  up.dipoleCorrection(b, dp, (int)b.size(), 6.28);\n"
                    "  return b.size() == 100 ? 0 : 1; }\n")
     exe = tmp_path / "all"
     r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-DSpherical", "-DFP64", "-DEXPANSION=10", f"-I{HOST}",
@@ -58,3 +60,34 @@ def test_reference_driver_runs_on_gpu(args):
     cp, ca = _errors(c.stdout)
     assert abs(gp - cp) <= 1e-6 * cp + 1e-13, (gp, cp)
     assert abs(ga - ca) <= 1e-6 * ca + 1e-13, (ga, ca)
+
+
+@pytest.mark.gpu
+def test_reference_ewald_driver_runs_on_gpu(tmp_path):
+    """oracle/ref_ewald.cxx (the single-rank sequence of examples/ewald.cxx:
+    periodic FMM, getDipole, dipoleCorrection, then Ewald wave, real and self
+    parts) compiled against the host API: its dumps match the reference's."""
+    import numpy as np
+    from helpers import GOLDEN, rel_l2
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_ewald_hip_p10")
+    if not os.path.exists(exe):
+        pytest.skip("the driver binary is built only where the reference sources exist")
+    z = dict(np.load(os.path.join(GOLDEN, "lattice12_p10_img3_ewald.npz"), allow_pickle=False))
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([z["input"].shape[0]], np.int32).tofile(f)
+        z["input"].tofile(f)
+    args = [str(v) for v in (z["theta"], z["ncrit"], z["nspawn"], z["images"], z["ksize"])] + \
+           [repr(float(z[k])) for k in ("alpha", "sigma", "cutoff", "cycle")]
+    print("[gpu] ewald driver on the host API", flush=True)
+    r = subprocess.run([exe, str(inp), str(out)] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    b = open(out, "rb").read()
+    n = int(np.frombuffer(b, np.int32, 1, 4)[0])
+    assert np.abs(np.frombuffer(b, np.float64, 3, 8) - z["dipole"]).max() < 1e-12
+    off = 32
+    for k in ("fmm_raw", "fmm", "ewald"):
+        v = np.frombuffer(b, np.float64, 4 * n, off).reshape(n, 4)
+        off += 32 * n
+        assert rel_l2(v[:, 0], z[k][:, 0]) < 1e-12, k
+        assert rel_l2(v[:, 1:], z[k][:, 1:]) < 1e-12, k

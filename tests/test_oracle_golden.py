@@ -13,7 +13,7 @@ from helpers import case_params, load_case, rel_l2
 
 CASES = ["cube4k_u_p10", "cube16k_u_p10", "cube16k_a_p10", "plummer16k_a_p10", "plummer16k_u_p10",
          "sphere16k_a_p8", "mass_plummer16k_a_p10", "mass_plummer8k_u_p8", "cube4k_u_p4", "cube4k_a_p12",
-         "octant8k_a_p6", "cube2k_u_p6_periodic", "unit16k_u_p10", "small100_u_p10", "tiny40_u_p10",
+         "octant8k_a_p6", "cube2k_u_p6_periodic", "mass_cube2k_u_p8_img3", "cube4k_a_p10_img3", "unit16k_u_p10", "small100_u_p10", "tiny40_u_p10",
          "cube4k_u_p10_mut", "cube16k_u_p10_mut", "plummer16k_u_p10_mut", "cube16k_a_p10_mut",
          "mass_plummer16k_a_p10_mut"]
 
