@@ -136,11 +136,14 @@ static int guarded(exafmm_hip_handle *H, F &&f) {
 extern "C" {
 
 static int check_config(const exafmm_hip_config *cfg) {
-  if (cfg->P != 4 && cfg->P != 6 && cfg->P != 8 && cfg->P != 10 && cfg->P != 12) return EXAFMM_HIP_EUNSUP;
+  if (cfg->basis == 0 && cfg->P != 4 && cfg->P != 6 && cfg->P != 8 && cfg->P != 10 && cfg->P != 12)
+    return EXAFMM_HIP_EUNSUP;
   if (!(cfg->theta > 0 && cfg->theta < 1) || cfg->ncrit <= 0 || cfg->images < 0 || cfg->images > 8)
     return EXAFMM_HIP_EINVAL;
   if (cfg->builder != 0 && cfg->builder != 1) return EXAFMM_HIP_EINVAL;
   if (cfg->precision != 0 && cfg->precision != 1) return EXAFMM_HIP_EINVAL;
+  if (cfg->basis != 0 && cfg->basis != 1) return EXAFMM_HIP_EINVAL;
+  if (cfg->basis == 1 && cfg->P != 4 && cfg->P != 6 && cfg->P != 8) return EXAFMM_HIP_EUNSUP;
   return EXAFMM_HIP_OK;
 }
 
@@ -151,7 +154,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   exafmm_hip_handle *H = new exafmm_hip_handle();
   H->h.cfg = *cfg;
   if (H->h.cfg.nspawn <= 0) H->h.cfg.nspawn = 1000;
-  H->h.nterm = cfg->P * (cfg->P + 1) / 2;
+  H->h.nterm = cfg->basis == 1 ? cfg->P * (cfg->P + 1) * (cfg->P + 2) / 6 : cfg->P * (cfg->P + 1) / 2;
   try {
     set_device(H->h);
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
@@ -178,7 +181,8 @@ const char *exafmm_hip_last_error(const exafmm_hip_handle *H) { return H ? H->h.
 int exafmm_hip_set_config(exafmm_hip_handle *H, const exafmm_hip_config *cfg) {
   return guarded(H, [&](Handle &h) {
     ensure(cfg != nullptr, "set_config: null config");
-    ensure(cfg->P == h.cfg.P && cfg->device == h.cfg.device, "set_config: P and device are fixed at creation");
+    ensure(cfg->P == h.cfg.P && cfg->device == h.cfg.device && cfg->basis == h.cfg.basis,
+           "set_config: P, basis and device are fixed at creation");
     if (int rc = check_config(cfg)) throw Error(rc, "set_config: invalid configuration");
     const bool tree_changed = cfg->ncrit != h.cfg.ncrit || cfg->builder != h.cfg.builder;
     h.cfg = *cfg;

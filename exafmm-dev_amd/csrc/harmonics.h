@@ -21,6 +21,18 @@ struct Expn {
 
 __host__ __device__ constexpr int tri(int n) { return n * (n + 1) / 2; }
 
+// |a - b| with the reference's rounding: norm() sums x^2, y^2, z^2 left to
+// right without contraction (vec.h:163-167), then std::sqrt.
+__device__ inline double exact_dist(double ax, double ay, double az, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  const double dx = ax - bx, dy = ay - by, dz = az - bz;
+  double t = 0;
+  t += dx * dx;
+  t += dy * dy;
+  t += dz * dz;
+  return sqrt(t);
+}
+
 struct Geom {
   double r;     // |dX|
   double ct;    // cos(alpha)

@@ -114,9 +114,10 @@ struct Handle {
   DBuf<double> ew_waves;
   DBuf<double4> ew_trg;
 
-  // M2L lane schedule (a pure function of P).
+  // M2L lane schedule (a pure function of P, the basis and the table type).
   DBuf<uint32_t> m2l_sched;
   int m2l_sched_P = -1;
+  int cart_maxc = 0;  // Cartesian schedule: chunks per output (outslot row length)
 
   // Scratch.
   DBuf<unsigned char> tmp;         // hipCUB temporary storage
@@ -146,6 +147,10 @@ void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
 void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg);
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
+void run_upward_cart(Handle &h);
+void run_downward_cart(Handle &h);
+void run_m2l_cart(Handle &h);
+void run_periodic_cart(Handle &h);
 void run_get_dipole(Handle &h, const double *X0, double *out);
 void run_dipole_correction(Handle &h, const double *dipole, int numBodies, double cycle);
 void run_ewald(Handle &h, int ksize, double alpha, double sigma, double cutoff, double cycle, int parts,

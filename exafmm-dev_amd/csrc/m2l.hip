@@ -318,6 +318,7 @@ static void m2l_impl(Handle &h) {
 
 void run_m2l(Handle &h) {
   if (h.nm2l_tgt == 0) return;
+  if (h.cfg.basis == 1) return run_m2l_cart(h);
   switch (h.cfg.P) {
     case 4: m2l_impl<4>(h); break;
     case 6: m2l_impl<6>(h); break;

@@ -12,18 +12,6 @@
 
 namespace exafmm {
 
-// |a - b| with the reference's rounding: norm() sums x^2, y^2, z^2 left to
-// right without contraction (vec.h:163-167), then std::sqrt.
-__device__ inline double exact_dist(double ax, double ay, double az, double bx, double by, double bz) {
-#pragma clang fp contract(off)
-  const double dx = ax - bx, dy = ay - by, dz = az - bz;
-  double t = 0;
-  t += dx * dx;
-  t += dy * dy;
-  t += dz * dz;
-  return sqrt(t);
-}
-
 template <int P>
 __global__ void __launch_bounds__(64)
     k_upward(const int *__restrict__ cells, const double4 *__restrict__ body, double4 *__restrict__ cX,
@@ -313,9 +301,13 @@ void run_upward(Handle &h) {
   h.M.resize((size_t)h.ncells * h.nterm);
   h.L.resize((size_t)h.ncells * h.nterm);
   EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
+  if (h.cfg.basis == 1) return run_upward_cart(h);
   EXA_DISPATCH_P(upward_impl, h);
 }
 
-void run_downward(Handle &h) { EXA_DISPATCH_P(downward_impl, h); }
+void run_downward(Handle &h) {
+  if (h.cfg.basis == 1) return run_downward_cart(h);
+  EXA_DISPATCH_P(downward_impl, h);
+}
 
 }  // namespace exafmm

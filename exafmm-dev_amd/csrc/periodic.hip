@@ -199,6 +199,12 @@ static void periodic_impl(Handle &h) {
 
 void run_periodic(Handle &h) {
   if (h.cfg.images <= 0 || h.ncells == 0) return;
+  if (h.cfg.basis == 1) {
+    run_periodic_cart(h);
+    if (h.cfg.mass) hipLaunchKernelGGL(k_periodic_mass_L, dim3(1), dim3(64), 0, h.stream, h.M.p, h.L.p, h.nterm);
+    EXA_HIP(hipGetLastError());
+    return;
+  }
   switch (h.cfg.P) {
     case 4: periodic_impl<4>(h); break;
     case 6: periodic_impl<6>(h); break;

@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("mass", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("basis", ctypes.c_int),
     ]
 
 
@@ -111,18 +112,20 @@ class FMM:
     """One particle system on one GPU (one handle, one HIP stream)."""
 
     def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, mutual=0, images=0,
-                 cycle=2 * np.pi, builder="uniform", mass=False, precision="fp64", device=0):
+                 cycle=2 * np.pi, builder="uniform", mass=False, precision="fp64", device=0, basis="spherical"):
         self.lib = load()
         b = {"uniform": 0, "adaptive": 1}[builder] if isinstance(builder, str) else int(builder)
         prec = {"fp64": 0, "mixed": 1}[precision] if isinstance(precision, str) else int(precision)
-        self.cfg = Config(P, theta, ncrit, nspawn, useRmax, useRopt, mutual, images, cycle, b, int(mass), prec, device)
+        bas = {"spherical": 0, "cartesian": 1}[basis] if isinstance(basis, str) else int(basis)
+        self.cfg = Config(P, theta, ncrit, nspawn, useRmax, useRopt, mutual, images, cycle, b, int(mass), prec, device,
+                          bas)
         h = ctypes.c_void_p()
         rc = self.lib.exafmm_hip_create(ctypes.byref(self.cfg), ctypes.byref(h))
         if rc != 0:
             raise ExafmmError(f"exafmm_hip_create failed with status {rc}")
         self.h = h
         self.P = P
-        self.nterm = P * (P + 1) // 2
+        self.nterm = P * (P + 1) * (P + 2) // 6 if bas == 1 else P * (P + 1) // 2
         self.n = 0
 
     def close(self):

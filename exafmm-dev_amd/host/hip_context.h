@@ -43,6 +43,11 @@ struct Context {
 #endif
     cfg.precision = 0;
     cfg.device = 0;
+#if Cartesian
+    cfg.basis = 1;  // kernels/LaplaceCartesianCPU.cxx
+#else
+    cfg.basis = 0;
+#endif
     check(exafmm_hip_create(&cfg, &h), "exafmm_hip_create");
   }
   ~Context() {

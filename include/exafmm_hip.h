@@ -30,7 +30,8 @@ typedef struct exafmm_hip_handle exafmm_hip_handle;
 
 /* Run-time configuration; field names follow include/args.h:98-100. */
 typedef struct {
-  int P;          /* expansion order, one of 4, 6, 8, 10, 12 (types.h:38)      */
+  int P;          /* expansion order (types.h:38): spherical 4, 6, 8, 10, 12;
+                     Cartesian 4, 6, 8                                         */
   double theta;   /* MAC parameter (args.h: -t)                                */
   int ncrit;      /* max bodies per leaf (args.h: -c)                          */
   int nspawn;     /* split-both threshold of the traversal (args.h: -s)        */
@@ -43,8 +44,13 @@ typedef struct {
   int builder;    /* 0: uniform Morton tree (include/build_tree_omp.h),
                      1: adaptive octree (include/build_tree_tbb.h)             */
   int mass;       /* 1: MASS kernel variants (LaplaceSphericalCPU.cxx:157,200) */
-  int precision;  /* 0: FP64 throughout; 1: mixed (FP32 P2P with Kahan)       */
+  int precision;  /* 0: FP64 throughout; 1: mixed (FP32 P2P with compensated
+                     sums; FP32 tables in the spherical M2L)                   */
   int device;     /* HIP device ordinal                                        */
+  int basis;      /* 0: spherical harmonics (kernels/LaplaceSphericalCPU.cxx,
+                     NTERM = P(P+1)/2 complex); 1: Cartesian Taylor series
+                     (kernels/LaplaceCartesianCPU.cxx, NTERM = P(P+1)(P+2)/6
+                     real, kept in the real parts of M and L)                 */
 } exafmm_hip_config;
 
 /* Phase timings (ms, HIP events) and work counts.  P2P runs on a second
@@ -103,8 +109,8 @@ int exafmm_hip_evaluate(exafmm_hip_handle *h);
 
 /* Readback.  Cells: ints5 = (IPARENT, ICHILD, NCHILD, IBODY, NBODY) per cell
  * in the reference's Cell order (include/types.h:79-92); X = 3 per cell;
- * R after the upward pass; M, L = 2*NTERM doubles per cell.  Any pointer may
- * be NULL. */
+ * R after the upward pass; M, L = 2*NTERM doubles per cell (re, im; the
+ * Cartesian basis has im = 0).  Any pointer may be NULL. */
 int exafmm_hip_get_cells(exafmm_hip_handle *h, int *ints5, uint64_t *icell, double *X, double *R, double *M,
                          double *L);
 /* Bodies in tree order: xyzq, original index (Body::IBODY) and results

@@ -142,6 +142,37 @@ void singular_harmonics(int P, double rho, double alpha, double beta, cd *Y) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Cartesian (Taylor) basis, kernels/LaplaceCartesianCPU.cxx.  Coefficient k =
+// (kx, ky, kz), |k| <= P-1, is stored at the index of Index<kx,ky,kz>::I
+// (:3-24): by degree, and within a degree by kx descending, then ky
+// descending.  M_k = sum q (X_c - x)^k / k!, C_k = d^k (1/r) (raw
+// derivatives), L_k = d^k phi at the cell centre.
+struct CartTables {
+  int P = 0, nt = 0;
+  std::vector<int> kx, ky, kz, deg;
+  std::vector<int> idx;  // (P+1)^3 lookup, -1 where |k| > P-1
+  int at(int a, int b, int c) const {
+    if (a < 0 || b < 0 || c < 0 || a + b + c > P - 1) return -1;
+    return idx[(a * (P + 1) + b) * (P + 1) + c];
+  }
+  void build(int P_) {
+    P = P_;
+    nt = P * (P + 1) * (P + 2) / 6;
+    idx.assign((P + 1) * (P + 1) * (P + 1), -1);
+    for (int n = 0; n < P; n++)
+      for (int a = n; a >= 0; a--)
+        for (int b = n - a; b >= 0; b--) {
+          const int c = n - a - b;
+          idx[(a * (P + 1) + b) * (P + 1) + c] = (int)kx.size();
+          kx.push_back(a);
+          ky.push_back(b);
+          kz.push_back(c);
+          deg.push_back(n);
+        }
+  }
+};
+
 struct orc_ctx {
   orc_params p;
   int nterm = 0;
@@ -158,6 +189,7 @@ struct orc_ctx {
   std::vector<int> p2p_i, p2p_j, p2p_img, m2l_i, m2l_j, m2l_img;
   int level = 0;
 
+  CartTables ct;  // Cartesian basis tables (p.basis == 1)
   cd *Mc(int c) { return &M[(size_t)c * nterm]; }
   cd *Lc(int c) { return &L[(size_t)c * nterm]; }
 
@@ -430,6 +462,7 @@ struct orc_ctx {
   // -------------------------------------------------------------------------
   // Kernels, LaplaceSphericalCPU.cxx:103-256.
   void P2M(int c) {  // :103-118
+    if (p.basis == 1) return c_P2M(c);
     const int P = p.P;
     std::vector<cd> Y(P * P), Yt(P * P);
     const OCell &C = cells[c];
@@ -446,6 +479,7 @@ struct orc_ctx {
   }
 
   void M2M(int ci) {  // :120-147
+    if (p.basis == 1) return c_M2M(ci);
     const int P = p.P;
     std::vector<cd> Y(P * P), Yt(P * P);
     const OCell &Ci = cells[ci];
@@ -477,6 +511,7 @@ struct orc_ctx {
   }
 
   void M2L(int ci, int cj, const double *Xp, bool mutual) {  // :149-191
+    if (p.basis == 1) return c_M2L(ci, cj, Xp, mutual);
     const int P = p.P;
     std::vector<cd> Y(P * P), Yj(P * P);
     const OCell &Ci = cells[ci], &Cj = cells[cj];
@@ -520,6 +555,7 @@ struct orc_ctx {
   }
 
   void L2L(int ci) {  // :193-224
+    if (p.basis == 1) return c_L2L(ci);
     const int P = p.P;
     std::vector<cd> Y(P * P), Yt(P * P);
     const OCell &Ci = cells[ci];
@@ -556,6 +592,7 @@ struct orc_ctx {
   }
 
   void L2P(int ci) {  // :226-256
+    if (p.basis == 1) return c_L2P(ci);
     const int P = p.P;
     std::vector<cd> Y(P * P), Yt(P * P);
     const OCell &C = cells[ci];
@@ -589,6 +626,217 @@ struct orc_ctx {
       T[1] += cart[0];
       T[2] += cart[1];
       T[3] += cart[2];
+    }
+  }
+
+  // -------------------------------------------------------------------------
+  // Cartesian kernels, LaplaceCartesianCPU.cxx:590-704.  Coefficients live in
+  // the real parts of M and L (imaginary parts stay 0).
+  static double &re(cd *a, int i) { return reinterpret_cast<double *>(a)[2 * i]; }
+  static double re(const cd *a, int i) { return reinterpret_cast<const double *>(a)[2 * i]; }
+
+  // Kernels::power (:183-186,213-216,243-246): C_k = C_{k-e} dX_d / k_d with
+  // e along z if kz > 0, else along y if ky > 0, else along x; C[0] preset.
+  void c_power(double *C, const double *dX) const {
+    for (int i = 1; i < ct.nt; i++) {
+      const int a = ct.kx[i], b = ct.ky[i], c = ct.kz[i];
+      if (c > 0)
+        C[i] = C[ct.at(a, b, c - 1)] * dX[2] / c;
+      else if (b > 0)
+        C[i] = C[ct.at(a, b - 1, 0)] * dX[1] / b;
+      else
+        C[i] = C[ct.at(a - 1, 0, 0)] * dX[0] / a;
+    }
+  }
+
+  // getCoef (:258-263): C_k = d^k (1/r) at dX, from the Taylor-coefficient
+  // recurrence a_k = -((2n-1) sum_d x_d a_{k-e_d} + (n-1) sum_d a_{k-2e_d})
+  // / (n r^2), n = |k| (Kernels::derivative, :187-191), then scaled by k!
+  // (Kernels::scale).  invR carries the MASS factor when present.
+  void c_coef(double *C, const double *dX, double invR2, double invR) const {
+    C[0] = invR;
+    for (int i = 1; i < ct.nt; i++) {
+      const int k[3] = {ct.kx[i], ct.ky[i], ct.kz[i]};
+      const int n = ct.deg[i];
+      double s1 = 0, s2 = 0;
+      for (int d = 0; d < 3; d++) {
+        int k1[3] = {k[0], k[1], k[2]};
+        k1[d] -= 1;
+        if (k1[d] >= 0) s1 += dX[d] * C[ct.at(k1[0], k1[1], k1[2])];
+        k1[d] -= 1;
+        if (k1[d] >= 0) s2 += C[ct.at(k1[0], k1[1], k1[2])];
+      }
+      C[i] = ((1 - 2 * n) * s1 + (1 - n) * s2) / n * invR2;
+    }
+    for (int i = 1; i < ct.nt; i++) {
+      double f = 1;
+      for (int t = 2; t <= ct.kx[i]; t++) f *= t;
+      for (int t = 2; t <= ct.ky[i]; t++) f *= t;
+      for (int t = 2; t <= ct.kz[i]; t++) f *= t;
+      C[i] *= f;
+    }
+  }
+
+  // sumM2L (:373-588): L_a += M_0 C_a (MASS: C_a) for every a, then
+  // L_a += sum_b M_b C_{a+b}.  For P <= 7 the reference's unrolled forms add
+  // the b terms in groups of |a| + |b| = g (g = 2..P-1; for a = 0 the group
+  // g holds |b| = g - 1, so |b| = P-1 never reaches L_0), each group summed
+  // in index order; from P = 8 on, the generic form adds every b >= 1 into
+  // L_0 one by one and then, for 1 <= |a| <= P-2, all b with
+  // 1 <= |b| <= P-1-|a| in index order as one sum.
+  void c_sumM2L(cd *L, const double *C, const cd *M) const {
+    const int P = p.P, nt = ct.nt;
+    for (int i = 0; i < nt; i++) re(L, i) += p.mass ? C[i] : re(M, 0) * C[i];
+    auto cidx = [&](int a, int b) {
+      return ct.at(ct.kx[a] + ct.kx[b], ct.ky[a] + ct.ky[b], ct.kz[a] + ct.kz[b]);
+    };
+    if (P <= 7) {
+      for (int g = 2; g <= P - 1; g++)
+        for (int a = 0; a < nt; a++) {
+          const int da = ct.deg[a];
+          const int db = da == 0 ? g - 1 : g - da;
+          if (da > g - 1 || db < 1) continue;
+          double sum = 0;
+          bool first = true;
+          for (int b = 1; b < nt; b++) {
+            if (ct.deg[b] != db) continue;
+            const double t = re(M, b) * C[cidx(a, b)];
+            sum = first ? t : sum + t;
+            first = false;
+          }
+          re(L, a) += sum;
+        }
+    } else {
+      for (int i = 1; i < nt; i++) re(L, 0) += re(M, i) * C[i];
+      for (int a = 1; a < nt; a++) {
+        const int da = ct.deg[a];
+        if (da > P - 2) continue;
+        double sum = 0;
+        for (int b = 1; b < nt; b++)
+          if (ct.deg[b] <= P - 1 - da) sum += re(M, b) * C[cidx(a, b)];
+        re(L, a) += sum;
+      }
+    }
+  }
+
+  void c_P2M(int c) {  // :590-598
+    const OCell &C = cells[c];
+    cd *Mv = Mc(c);
+    std::vector<double> m(ct.nt);
+    for (int b = C.ibody; b < C.ibody + C.nbody; b++) {
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = C.X[d] - X[3 * (size_t)b + d];
+      m[0] = Q[b];
+      c_power(m.data(), dX);
+      for (int i = 0; i < ct.nt; i++) re(Mv, i) += m[i];
+    }
+  }
+
+  // Kernels::M2M / MultipoleSum (:109-133,192-195): M_n += C_{n-k} M_k over
+  // the box 0 <= k <= n, k != 0, in lexicographic (kx, ky, kz) order.
+  void c_M2M(int ci) {  // :600-612
+    const OCell &Ci = cells[ci];
+    cd *Mi = Mc(ci);
+    std::vector<double> C(ct.nt);
+    for (int cj = Ci.ichild; cj < Ci.ichild + Ci.nchild; cj++) {
+      const cd *Mj = Mc(cj);
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - cells[cj].X[d];
+      C[0] = 1;
+      c_power(C.data(), dX);
+      c_m2m_add(Mi, C.data(), Mj);
+    }
+  }
+
+  void c_m2m_add(cd *Mi, const double *C, const cd *Mj) const {
+    for (int i = 0; i < ct.nt; i++) re(Mi, i) += C[i] * re(Mj, 0);
+    for (int n = 1; n < ct.nt; n++) {
+      const int nx = ct.kx[n], ny = ct.ky[n], nz = ct.kz[n];
+      double sum = 0;
+      for (int a = 0; a <= nx; a++)
+        for (int b = 0; b <= ny; b++)
+          for (int c = 0; c <= nz; c++) {
+            if (a == 0 && b == 0 && c == 0) continue;
+            sum += C[ct.at(nx - a, ny - b, nz - c)] * re(Mj, ct.at(a, b, c));
+          }
+      re(Mi, n) += sum;
+    }
+  }
+
+  void c_M2L(int ci, int cj, const double *Xp, bool mutual) {  // :614-629
+    const OCell &Ci = cells[ci], &Cj = cells[cj];
+    double dX[3];
+    for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - Cj.X[d] - Xp[d];
+    const double invR2 = 1 / norm3(dX);
+    const double invR = p.mass ? re(Mc(ci), 0) * re(Mc(cj), 0) * std::sqrt(invR2) : std::sqrt(invR2);
+    std::vector<double> C(ct.nt);
+    c_coef(C.data(), dX, invR2, invR);
+    c_sumM2L(Lc(ci), C.data(), Mc(cj));
+    if (mutual) {  // Coefs::negate (:578-588): odd degrees flip sign
+      for (int i = 0; i < ct.nt; i++)
+        if (ct.deg[i] & 1) C[i] = -C[i];
+      c_sumM2L(Lc(cj), C.data(), Mc(ci));
+    }
+  }
+
+  // L2L (:631-645): L_child = L_parent shifted by the child offset.
+  void c_L2L(int ci) {
+    const OCell &Ci = cells[ci];
+    const int cj = Ci.iparent;
+    double dX[3];
+    for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - cells[cj].X[d];
+    std::vector<double> C(ct.nt);
+    C[0] = 1;
+    c_power(C.data(), dX);
+    cd *Li = Lc(ci);
+    const cd *Lj = Lc(cj);
+    if (p.mass) {
+      const double m0 = re(Mc(ci), 0);
+      for (int i = 0; i < ct.nt; i++) re(Li, i) /= m0;
+    }
+    for (int i = 0; i < ct.nt; i++) re(Li, i) += re(Lj, i);
+    for (int i = 1; i < ct.nt; i++) re(Li, 0) += C[i] * re(Lj, i);
+    c_local_sum_into(Li, C.data(), Lj, ct.P - 1);
+  }
+
+  // Kernels::L2L / L2P with LocalSum (:139-166): for every n with
+  // 1 <= |n| <= nmax, out_n += sum_k C_k Lsrc_{n+k}, 1 <= |k| <= P-1-|n|,
+  // in index order.
+  void c_local_sum_into(cd *out, const double *C, const cd *Lsrc, int nmax) const {
+    for (int n = 1; n < ct.nt; n++) {
+      if (ct.deg[n] > nmax) continue;
+      double sum = 0;
+      for (int k = 1; k < ct.nt; k++) {
+        const int j = ct.at(ct.kx[n] + ct.kx[k], ct.ky[n] + ct.ky[k], ct.kz[n] + ct.kz[k]);
+        if (j >= 0) sum += C[k] * re(Lsrc, j);
+      }
+      re(out, n) += sum;
+    }
+  }
+
+  void c_L2P(int ci) {  // :647-664
+    const OCell &C = cells[ci];
+    const cd *Lv = Lc(ci);
+    std::vector<double> Cc(ct.nt);
+    for (int b = C.ibody; b < C.ibody + C.nbody; b++) {
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = X[3 * (size_t)b + d] - C.X[d];
+      Cc[0] = 1;
+      c_power(Cc.data(), dX);
+      double *T = &trg[4 * (size_t)b];
+      for (int d = 0; d < 4; d++) T[d] /= Q[b];
+      for (int d = 0; d < 4; d++) T[d] += re(Lv, d);
+      for (int i = 1; i < ct.nt; i++) T[0] += Cc[i] * re(Lv, i);
+      for (int d = 0; d < 3; d++) {  // n = e_x, e_y, e_z (indices 1, 2, 3)
+        double sum = 0;
+        for (int k = 1; k < ct.nt; k++) {
+          int e[3] = {0, 0, 0};
+          e[d] = 1;
+          const int j = ct.at(e[0] + ct.kx[k], e[1] + ct.ky[k], e[2] + ct.kz[k]);
+          if (j >= 0) sum += Cc[k] * re(Lv, j);
+        }
+        T[d + 1] += sum;
+      }
     }
   }
 
@@ -833,6 +1081,13 @@ struct orc_ctx {
             const double Xc[3] = {Cp.X[0] + ix * cycle, Cp.X[1] + iy * cycle, Cp.X[2] + iz * cycle};
             double dX[3];
             for (int d = 0; d < 3; d++) dX[d] = Cp.X[d] - Xc[d];
+            if (p.basis == 1) {
+              std::vector<double> C(ct.nt);
+              C[0] = 1;
+              c_power(C.data(), dX);
+              c_m2m_add(Mn.data(), C.data(), Mp.data());
+              continue;
+            }
             double rho, alpha, beta;
             cart2sph(rho, alpha, beta, dX);
             regular_harmonics(P, rho, alpha, beta, Y.data(), Yt.data());
@@ -870,6 +1125,16 @@ struct orc_ctx {
 
   // M2L against an explicit source expansion (periodic parent).
   void m2l_generic(int ci, const double *Xi, const double *Xj, const cd *Mj, const double *Xp) {
+    if (p.basis == 1) {  // LaplaceCartesianCPU.cxx:614-623 against an explicit source
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = Xi[d] - Xj[d] - Xp[d];
+      const double invR2 = 1 / norm3(dX);
+      const double invR = p.mass ? re(Mc(ci), 0) * re(Mj, 0) * std::sqrt(invR2) : std::sqrt(invR2);
+      std::vector<double> C(ct.nt);
+      c_coef(C.data(), dX, invR2, invR);
+      c_sumM2L(Lc(ci), C.data(), Mj);
+      return;
+    }
     const int P = p.P;
     std::vector<cd> Y(P * P);
     double dX[3];
@@ -1079,7 +1344,12 @@ orc_ctx *orc_create(const orc_params *p) {
   orc_ctx *c = new orc_ctx();
   c->p = *p;
   if (c->p.nspawn <= 0) c->p.nspawn = 1000;
-  c->nterm = p->P * (p->P + 1) / 2;
+  if (c->p.basis == 1) {
+    c->ct.build(p->P);
+    c->nterm = c->ct.nt;
+  } else {
+    c->nterm = p->P * (p->P + 1) / 2;
+  }
   return c;
 }
 

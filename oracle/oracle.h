@@ -32,6 +32,9 @@ typedef struct {
   int mass;       /* 1: MASS kernel variants (all charges positive masses)   */
   int threads;    /* OpenMP threads for the evaluation (0 = default)         */
   int mutual;     /* mutual traversal and kernels (traversal.h:199-212)      */
+  int basis;      /* 0 spherical (LaplaceSphericalCPU.cxx), 1 Cartesian       *
+                   * (LaplaceCartesianCPU.cxx, NTERM = P(P+1)(P+2)/6, real   *
+                   * coefficients kept in the real parts of M and L)        */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

@@ -26,6 +26,7 @@ class Params(ctypes.Structure):
         ("mass", ctypes.c_int),
         ("threads", ctypes.c_int),
         ("mutual", ctypes.c_int),
+        ("basis", ctypes.c_int),
     ]
 
 
@@ -80,9 +81,10 @@ class Oracle:
     STAGES_ALL = 31
 
     def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, images=0,
-                 cycle=2 * np.pi, adaptive=False, mass=False, threads=0, mutual=False):
+                 cycle=2 * np.pi, adaptive=False, mass=False, threads=0, mutual=False, basis="spherical"):
+        b = {"spherical": 0, "cartesian": 1}[basis] if isinstance(basis, str) else int(basis)
         self.params = Params(P, theta, ncrit, nspawn, useRmax, useRopt, images, cycle, int(adaptive), int(mass),
-                             threads, int(mutual))
+                             threads, int(mutual), b)
         self.h = lib().orc_create(ctypes.byref(self.params))
         self.P = P
 

@@ -48,6 +48,15 @@ CASES = [
     ("plummer16k_u_p10_mut", "harness_u_p10", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
     ("cube16k_a_p10_mut", "harness_a_p10", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 1, 0], "tree"),
     ("mass_plummer16k_a_p10_mut", "harness_a_p10_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 1, 0], "tree"),
+    # Cartesian (Taylor) basis, kernels/LaplaceCartesianCPU.cxx.
+    ("cube16k_u_p6_cart", "harness_u_p6_cart", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 0, 0], "tree"),
+    ("cube16k_a_p6_cart", "harness_a_p6_cart", ["gen", "c", "16384"], [0.4, 64, 1000, 1, 0, 0, 0], "tree"),
+    ("cube4k_u_p4_cart", "harness_u_p4_cart", ["gen", "c", "4096"], [0.4, 64, 1000, 1, 0, 0, 0], "full"),
+    ("sphere8k_a_p8_cart", "harness_a_p8_cart", ["gen", "s", "8192"], [0.5, 32, 1000, 1, 0, 0, 0], "tree"),
+    ("mass_plummer16k_a_p6_cart", "harness_a_p6_cart_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 0, 0],
+     "tree"),
+    ("cube4k_u_p6_cart_mut", "harness_u_p6_cart", ["gen", "c", "4096"], [0.4, 64, 1000, 1, 0, 1, 0], "full"),
+    ("cube2k_u_p6_cart_img3", "harness_u_p6_cart", ["gen", "c", "2048"], [0.4, 64, 1000, 1, 0, 0, 3], "tree"),
 ]
 
 
@@ -61,6 +70,8 @@ EWALD_CASES = [
      [0.35, 64, 1000, 2, 11, 10 / CYCLE, 0.25 / np.pi, 10.0, CYCLE]),
     ("cube2k_p10_img4_ewald", "ref_ewald_p10", ("cube", 2048, 3),
      [0.4, 32, 1000, 4, 11, 10 / CYCLE, 0.25 / np.pi, CYCLE / 2, CYCLE]),
+    ("lattice12_p6_cart_img3_ewald", "ref_ewald_p6_cart", ("lattice", 12, 5),
+     [0.35, 64, 1000, 3, 11, 10 / CYCLE, 0.25 / np.pi, CYCLE / 2, CYCLE]),
 ]
 
 
@@ -130,6 +141,7 @@ def main():
                      err_pot=r["err_pot"], err_acc=r["err_acc"], xmin=r["xmin"], xmax=r["xmax"],
                      ibody=r["ibody"], key=r["key"], sample_ibody=r["sample_ibody"],
                      sample_direct=r["sample_direct"], adaptive="_a_" in name, mass=name.startswith("mass"),
+                     basis=1 if "_cart" in name else 0,
                      c_iparent=c["iparent"], c_ichild=c["ichild"], c_nchild=c["nchild"], c_ibody=c["ibody"],
                      c_nbody=c["nbody"], c_icell=c["icell"], c_X=c["X"], c_Rbuild=c["Rbuild"], c_R=c["R"])
             if mode[0] == "file":
@@ -164,7 +176,8 @@ def main():
             r = read_ewald_dump(out)
             np.savez_compressed(os.path.join(OUT, name + ".npz"), input=x, theta=opts[0], ncrit=opts[1],
                                 nspawn=opts[2], images=opts[3], ksize=opts[4], alpha=opts[5], sigma=opts[6],
-                                cutoff=opts[7], cycle=opts[8], P=int(binary.rsplit("_p", 1)[1]), **r)
+                                cutoff=opts[7], cycle=opts[8], P=int(binary.split("_p")[1].split("_")[0]),
+                                basis=1 if binary.endswith("_cart") else 0, **r)
     # Generator fixtures: drand48 and the first bodies of every distribution.
     g = dict(drand48_seed0=O.drand48_seq(0, 16), drand48_seed42=O.drand48_seq(42, 16))
     for dist in "cspol":

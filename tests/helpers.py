@@ -19,7 +19,8 @@ def load_case(name):
 def case_params(z):
     return dict(P=int(z["P"]), theta=float(z["theta"]), ncrit=int(z["ncrit"]), nspawn=int(z["nspawn"]),
                 useRmax=int(z["useRmax"]), useRopt=int(z["useRopt"]), images=int(z["images"]),
-                adaptive=bool(z["adaptive"]), mass=bool(z["mass"]), mutual=bool(z["mutual"]))
+                adaptive=bool(z["adaptive"]), mass=bool(z["mass"]), mutual=bool(z["mutual"]),
+                basis="cartesian" if int(z.get("basis", 0)) == 1 else "spherical")
 
 
 def unit_cube(n, seed):

@@ -18,7 +18,7 @@ import pytest
 import oracle as O
 from helpers import GOLDEN, rel_l2
 
-CASES = ["lattice12_p10_img3_ewald", "lattice12_p6_img2_ewald", "cube2k_p10_img4_ewald"]
+CASES = ["lattice12_p10_img3_ewald", "lattice12_p6_img2_ewald", "cube2k_p10_img4_ewald", "lattice12_p6_cart_img3_ewald"]
 MADELUNG_NACL = 1.747564594633182
 
 
@@ -38,15 +38,20 @@ def test_periodic_fmm_dipole_ewald_bitexact(name):
     z = load(name)
     x = z["input"]
     cyc = float(z["cycle"])
+    cart = int(z.get("basis", 0)) == 1
     o = O.Oracle(P=int(z["P"]), theta=float(z["theta"]), ncrit=int(z["ncrit"]), nspawn=int(z["nspawn"]),
-                 images=int(z["images"]), cycle=cyc)
+                 images=int(z["images"]), cycle=cyc, basis=int(z.get("basis", 0)))
     o.set_bodies(x)
     o.run()
-    assert np.array_equal(caller_order(o), z["fmm_raw"])
+    # Spherical: bit-exact.  Cartesian: the derivative tensor comes from the
+    # Taylor recurrence instead of the reference's unrolled closed forms, so
+    # it agrees to rounding.
+    tol = 1e-13 if cart else 0.0
+    assert rel_l2(caller_order(o), z["fmm_raw"]) <= tol
     d = o.dipole()
     assert np.array_equal(d, z["dipole"])
     o.dipole_correction(d, x.shape[0], cyc)
-    assert np.array_equal(caller_order(o), z["fmm"])
+    assert rel_l2(caller_order(o), z["fmm"]) <= tol
     e = o.ewald(int(z["ksize"]), float(z["alpha"]), float(z["sigma"]), float(z["cutoff"]), cyc)
     assert np.array_equal(e, z["ewald"])
     # FMM vs Ewald as examples/ewald.cxx:136-148 reports them (the
