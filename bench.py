@@ -126,19 +126,24 @@ def main():
                  "phases_ms": {k: round(v, 4) for k, v in mmed.items()},
                  "dtype": "f32 P2P (compensated) and f32 M2L tables; f64 centres, offsets, expansions"}
 
+    # Per-kernel rates (algorithmic work / HIP-event time of that kernel).
     t_m2l, t_p2p = med["t_m2l"], med["t_p2p"]
-    m2l_flops = s["num_m2l_pairs"] * m2l_flops_per_pair(P)
+    t_mfma, t_pairs = med["t_m2l_mfma"], med["t_m2l_pairs"]
+    n_lat = s["num_m2l_lattice"]
+    n_irr = s["num_m2l_pairs"] - n_lat
+    fpp = m2l_flops_per_pair(P)
     p2p_pairs = s["num_p2p_bodies"]
-    p2p_flops = 20 * p2p_pairs
-    m2l_tf = m2l_flops / (t_m2l * 1e-3) / 1e12
-    p2p_tf = p2p_flops / (t_p2p * 1e-3) / 1e12
-    if t_m2l >= t_p2p:
-        dom = {"kernel": "k_m2l<%d,double>" % P, "bound": "mfma", "achieved": m2l_tf, "ms": t_m2l}
-    else:
-        dom = {"kernel": "k_p2p_f64", "bound": "mfma", "achieved": p2p_tf, "ms": t_p2p}
-    roofline = {"bound": dom["bound"], "achieved": round(dom["achieved"], 4), "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(dom["achieved"] / PEAK_FP64_TFLOPS, 5), "traffic": None,
-                "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 4)}
+    kern = {
+        "k_m2l_mfma<%d,double>" % P: (n_lat * fpp, t_mfma, "mfma", PEAK_FP64_TFLOPS),
+        "k_m2l<%d,double>" % P: (n_irr * fpp, t_pairs, "mfma", PEAK_FP64_TFLOPS),
+        "k_p2p_f64": (20 * p2p_pairs, t_p2p, "mfma", PEAK_FP64_TFLOPS),
+    }
+    kernels_tflops = {k: round(w / (t * 1e-3) / 1e12, 4) for k, (w, t, _, _) in kern.items() if t > 0}
+    dom_name = max((k for k in kern if kern[k][1] > 0), key=lambda k: kern[k][1])
+    w, t, bound, peak = kern[dom_name]
+    ach = w / (t * 1e-3) / 1e12
+    roofline = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 5), "traffic": None, "kernel": dom_name, "kernel_ms": round(t, 4)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, P)
     line = {
@@ -157,11 +162,13 @@ def main():
         "config": {"workload": "plain FP64 GPU FMM evaluation, Laplace spherical", "N": n, "P": P, "theta": 0.4,
                    "ncrit": 64, "tree": "uniform Morton (build_tree_omp.h)", "precision": "fp64"},
         "p2p_ginter_per_s": p2p_pairs / (t_p2p * 1e-3) / 1e9,
+        "m2l": {"pairs": s["num_m2l_pairs"], "lattice_pairs": n_lat, "offsets": s["num_m2l_offsets"],
+                "tiles": s["num_m2l_tiles"]},
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
         "phases_ms": {k: round(v, 4) for k, v in med.items()},
         "counts": {"cells": s["num_cells"], "p2p_cell_pairs": s["num_p2p_pairs"],
                    "m2l_cell_pairs": s["num_m2l_pairs"], "p2p_body_pairs": p2p_pairs},
-        "kernels_tflops": {"m2l": round(m2l_tf, 4), "p2p": round(p2p_tf, 4)},
+        "kernels_tflops": kernels_tflops,
         "roofline": roofline,
         "mixed": mixed,
         "cpu_baseline": cpu,

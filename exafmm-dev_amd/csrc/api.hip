@@ -2,6 +2,7 @@
 // point converts exceptions into status codes and keeps the message on the
 // handle.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.h"
@@ -154,6 +155,8 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   exafmm_hip_handle *H = new exafmm_hip_handle();
   H->h.cfg = *cfg;
   if (H->h.cfg.nspawn <= 0) H->h.cfg.nspawn = 1000;
+  // EXAFMM_HIP_M2L=pairs keeps every M2L pair on the per-pair kernel.
+  if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
   H->h.nterm = cfg->basis == 1 ? cfg->P * (cfg->P + 1) * (cfg->P + 2) / 6 : cfg->P * (cfg->P + 1) / 2;
   try {
     set_device(H->h);
@@ -270,6 +273,8 @@ int exafmm_hip_evaluate(exafmm_hip_handle *H) {
     s.t_traverse = elapsed(h.ev[4], h.ev[7]);
     s.t_downward = elapsed(h.ev[7], h.ev[8]);
     s.t_total = elapsed(h.ev[0], h.ev[8]);
+    s.t_m2l_mfma = h.lat_timed ? elapsed(h.ev[12], h.ev[13]) : 0;
+    s.t_m2l_pairs = h.pair_timed ? elapsed(h.ev[14], h.ev[15]) : 0;
   });
 }
 
@@ -413,6 +418,9 @@ int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
     h.stats.num_m2l_pairs = h.nm2l;
     h.stats.num_cells = h.ncells;
     h.stats.num_levels = h.nlevels;
+    h.stats.num_m2l_lattice = h.lat_stats_pairs;
+    h.stats.num_m2l_offsets = h.lat_stats_noff;
+    h.stats.num_m2l_tiles = h.lat_stats_tiles;
     if (h.have_trav) h.stats.num_p2p_bodies = count_p2p_body_pairs(h);
     *s = h.stats;
   });

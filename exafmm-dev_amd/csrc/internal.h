@@ -56,6 +56,10 @@ struct DBuf {
     cap = n = 0;
   }
   size_t bytes() const { return n * sizeof(T); }
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
 };
 
 // Double-precision complex stored as two doubles (re, im).
@@ -114,6 +118,20 @@ struct Handle {
   DBuf<double> ew_waves;
   DBuf<double4> ew_trg;
 
+  // Lattice M2L (offset-grouped MFMA contractions) and the CSR of the
+  // remaining (irregular) pairs.
+  bool lattice_m2l = true;
+  DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
+  DBuf<double> lat_part;
+  DBuf<int> lat_pkey, lat_hist, lat_flag, lat_kscan, lat_okey, lat_ookey, lat_tile_tgt, lat_tile_of, lat_col_of, lat_slot, lat_irr,
+      lat_scan, irr_off, irr_src, irr_tgt;
+  DBuf<unsigned char> irr_img;
+  DBuf<double> lat_tile_w, lat_T;
+  DBuf<float> lat_Tf;
+  int64_t lat_stats_pairs = 0;
+  int lat_stats_noff = 0, lat_stats_tiles = 0;
+  bool lat_timed = false, pair_timed = false;  // ev[12..13]: MFMA kernel, ev[14..15]: per-pair kernel
+
   // M2L lane schedule (a pure function of P, the basis and the table type).
   DBuf<uint32_t> m2l_sched;
   int m2l_sched_P = -1;
@@ -134,6 +152,15 @@ struct Handle {
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// A CSR interaction list by target cell (targets with work listed in tgt).
+struct M2LList {
+  const int *tgt;
+  int ntgt;
+  const int *off;
+  const int *src;
+  const unsigned char *img;
+};
+
 // Phase implementations (one translation unit each).
 void run_bounds(Handle &h);
 void run_build_uniform(Handle &h);
@@ -147,6 +174,7 @@ void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
 void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg);
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
+bool run_m2l_lattice(Handle &h, M2LList &irr);
 void run_upward_cart(Handle &h);
 void run_downward_cart(Handle &h);
 void run_m2l_cart(Handle &h);

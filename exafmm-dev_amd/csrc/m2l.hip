@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64)
 }
 
 template <int P>
-static void m2l_impl(Handle &h) {
+static void m2l_impl(Handle &h, const M2LList &lst) {
   using S = M2LShape<P>;
   const int esz = h.cfg.precision == 1 ? 8 : 16;  // bytes of one table entry
   const int key = P * 100 + esz;
@@ -307,24 +307,32 @@ static void m2l_impl(Handle &h) {
   const uint32_t *desc = h.m2l_sched.p;
   const int *slot0 = reinterpret_cast<const int *>(desc + 64 * S::IPL);
   const int *outslot = slot0 + 64;
+  if (lst.ntgt == 0) return;
+  EXA_HIP(hipEventRecord(h.ev[14], h.stream));
+  h.pair_timed = true;
   if (esz == 8)
-    hipLaunchKernelGGL((k_m2l<P, float>), dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p,
-                       h.m2l_src.p, h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+    hipLaunchKernelGGL((k_m2l<P, float>), dim3(lst.ntgt), dim3(64), 0, h.stream, lst.tgt, lst.off, lst.src, lst.img,
+                       h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
   else
-    hipLaunchKernelGGL((k_m2l<P, double>), dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p,
-                       h.m2l_src.p, h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+    hipLaunchKernelGGL((k_m2l<P, double>), dim3(lst.ntgt), dim3(64), 0, h.stream, lst.tgt, lst.off, lst.src, lst.img,
+                       h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+  EXA_HIP(hipEventRecord(h.ev[15], h.stream));
   EXA_HIP(hipGetLastError());
 }
 
 void run_m2l(Handle &h) {
   if (h.nm2l_tgt == 0) return;
   if (h.cfg.basis == 1) return run_m2l_cart(h);
+  // Lattice pairs on the matrix cores first; the rest per pair.
+  M2LList lst{h.m2l_tgt.p, h.nm2l_tgt, h.m2l_off.p, h.m2l_src.p, h.m2l_img.p};
+  h.pair_timed = false;
+  run_m2l_lattice(h, lst);
   switch (h.cfg.P) {
-    case 4: m2l_impl<4>(h); break;
-    case 6: m2l_impl<6>(h); break;
-    case 8: m2l_impl<8>(h); break;
-    case 10: m2l_impl<10>(h); break;
-    case 12: m2l_impl<12>(h); break;
+    case 4: m2l_impl<4>(h, lst); break;
+    case 6: m2l_impl<6>(h, lst); break;
+    case 8: m2l_impl<8>(h, lst); break;
+    case 10: m2l_impl<10>(h, lst); break;
+    case 12: m2l_impl<12>(h, lst); break;
     default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
   }
 }

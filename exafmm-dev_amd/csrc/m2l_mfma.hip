@@ -1,0 +1,605 @@
+// M2L of lattice pairs as dense contractions on the matrix cores.
+//
+// Reference: kernels/LaplaceSphericalCPU.cxx:149-191 (non-MASS).  Octree cell
+// centres of one level lie on that level's lattice, so the offset between a
+// target of width w and a source of the same level or one level up or down
+// is X_i - X_j = d u with d an integer vector and u = w / 4.  The singular
+// harmonics are homogeneous, S_a(u d) = u^-(a+1) S_a(d), so with
+// M~_n = M_n u^-n the translation of every such pair is one fixed real matrix
+// per offset d:  L_j = u^-(j+1) T(d) M~.  In real form (rows (j, k, re|im),
+// columns (n, m, re|im), m >= 0) the reference's sum over m = -n..n folds
+// into 2 x 2 blocks, with M'_n^m = (-1)^m M_n^m, M'_n^-m = conj M_n^m and the
+// signed harmonics S' of the spherical kernel:
+//   T[(j,k,re),(n,m,re)] = (-1)^j ((-1)^m S1r + [m>0] S2r)
+//   T[(j,k,re),(n,m,im)] = (-1)^j (-(-1)^m S1i + [m>0] S2i)
+//   T[(j,k,im),(n,m,re)] = (-1)^j ((-1)^m S1i + [m>0] S2i)
+//   T[(j,k,im),(n,m,im)] = (-1)^j ((-1)^m S1r - [m>0] S2r)
+// S1 = S'_{j+n}^{m-k}(d), S2 = S'_{j+n}^{-m-k}(d), zero unless j + n < P.
+//
+// Work split: targets are grouped into tiles of 64 cells of one level and
+// one octant (cells of one octant share most of their interaction offsets,
+// so a tile's columns are mostly busy at every offset);
+// a tile's local expansions (112 rows x 64 columns) stay in MFMA
+// accumulators while the workgroup walks the offsets: per offset the matrix
+// T(d) is staged in LDS in MFMA fragment order (only the 16 x 4 blocks that
+// are not identically zero: j + n < P), each lane gathers its column's source
+// multipole scaled by u^-n, and the 4 waves (16 columns each) run
+// v_mfma_f64_16x16x4 (v_mfma_f32_16x16x4 in the mixed form).  Pairs that are
+// not lattice pairs (levels two or more apart, periodic images, MASS mode,
+// offsets used by fewer than kLatMinPairs pairs) go to the per-pair kernel.
+#include <algorithm>
+#include <vector>
+
+#include "harmonics.h"
+#include "internal.h"
+
+namespace exafmm {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr double kLatUnit = 4;                   // u = w / kLatUnit
+constexpr int kLatD = 60;                         // |d_c| <= kLatD (in units of u)
+constexpr int kLatSide = 2 * kLatD + 1;
+constexpr int kLatKeys = kLatSide * kLatSide * kLatSide;
+constexpr int kLatMinPairs = 64;                  // offsets with fewer pairs go to the per-pair kernel
+
+template <int P>
+struct MfmaShape {
+  static constexpr int NT = P * (P + 1) / 2;
+  static constexpr int NR = 2 * NT;              // real rows / columns
+  static constexpr int RB = (NR + 15) / 16;      // row blocks
+  static constexpr int KC = (NR + 3) / 4;        // column chunks
+  static constexpr int jmin(int rb) {
+    int r = rb * 16 / 2, j = 0;
+    while ((j + 1) * (j + 2) / 2 <= r) j++;
+    return j;
+  }
+  static constexpr int kc_end(int rb) {
+    const int j = jmin(rb);
+    const int cols = 2 * ((P - j) * (P - j + 1) / 2);  // columns with n <= P-1-j
+    return (cols + 3) / 4;
+  }
+  static constexpr int base(int rb) {
+    int s = 0;
+    for (int i = 0; i < rb; i++) s += kc_end(i);
+    return s;
+  }
+  static constexpr int NBLK = base(RB);
+};
+
+// n of packed coefficient t = n(n+1)/2 + m.
+template <int P>
+struct DegreeTable {
+  int n[P * (P + 1) / 2 + 1];
+  constexpr DegreeTable() : n() {
+    int t = 0;
+    for (int a = 0; a < P; a++)
+      for (int b = 0; b <= a; b++) n[t++] = a;
+    n[t] = P;
+  }
+};
+
+// Component r of lane l of the 16x16x4 MFMA result holds D[row][l%16] with
+// row = (l/16) + 4 r (FP64) or 4 (l/16) + r (FP32); A[l%16][l/16] and
+// B[l/16][l%16] are lane l's operands for both.
+template <typename R>
+__device__ inline int d_row(int lane, int r) {
+  return sizeof(R) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+}
+
+template <typename R>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef v4d type;
+  static __device__ inline v4d mac(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef v4f type;
+  static __device__ inline v4f mac(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+// ---------------------------------------------------------------------------
+// Pair classification: key of the lattice offset, or -1.
+__global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *__restrict__ src,
+                               const unsigned char *__restrict__ img, int64_t np, const double4 *__restrict__ cX,
+                               const double *__restrict__ cRb, int *__restrict__ pkey, int *__restrict__ hist) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  int lo = 0, hi = nc;  // largest c with off[c] <= e
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int t = lo, s = src[e];
+  int key = -1;
+  if (img[e] == 13) {
+    const double u = 2 * cRb[t] / kLatUnit;
+    const double4 a = cX[t], b = cX[s];
+    const double dx = (a.x - b.x) / u, dy = (a.y - b.y) / u, dz = (a.z - b.z) / u;
+    const double rx = rint(dx), ry = rint(dy), rz = rint(dz);
+    if (fabs(dx - rx) < 1e-6 && fabs(dy - ry) < 1e-6 && fabs(dz - rz) < 1e-6 && fabs(rx) <= kLatD &&
+        fabs(ry) <= kLatD && fabs(rz) <= kLatD)
+      key = (((int)rx + kLatD) * kLatSide + ((int)ry + kLatD)) * kLatSide + ((int)rz + kLatD);
+  }
+  pkey[e] = key;
+  if (key >= 0) atomicAdd(&hist[key], 1);
+}
+
+// T(d) for every used offset, in fragment order: block (rb, kc) holds
+// T[rb*16 + l%16][kc*4 + l/16] at lane l.
+template <int P, typename R>
+__global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R *__restrict__ T) {
+  using S = MfmaShape<P>;
+  constexpr int NT = S::NT;
+  __shared__ cplx sS[P * P];
+  const int o = blockIdx.x, lane = threadIdx.x;
+  const int key = okey[o];
+  const double dx = key / (kLatSide * kLatSide) - kLatD, dy = (key / kLatSide) % kLatSide - kLatD,
+               dz = key % kLatSide - kLatD;
+  const Geom g = make_geom(dx, dy, dz);
+  for (int t = lane; t < NT; t += 64) {
+    int a, b;
+    coef_nm(t, a, b);
+    double re, im;
+    singular_nm(a, b, g, re, im);
+    const double sg = (b & 1) ? -1.0 : 1.0;  // S'_a^b = (-1)^b S_a^b (b > 0)
+    sS[a * a + a + b] = {sg * re, sg * im};
+    if (b > 0) sS[a * a + a - b] = {re, -im};  // S'_a^-b = S_a^-b = conj S_a^b
+  }
+  __syncthreads();
+  for (int rb = 0; rb < S::RB; rb++)
+    for (int kc = 0; kc < S::kc_end(rb); kc++) {
+      const int row = rb * 16 + (lane & 15), col = kc * 4 + (lane >> 4);
+      double v = 0;
+      if (row < S::NR && col < S::NR) {
+        int j, k, n, m;
+        coef_nm(row >> 1, j, k);
+        coef_nm(col >> 1, n, m);
+        if (j + n < P) {
+          const int a = j + n;
+          const cplx S1 = sS[a * a + a + m - k];
+          const cplx S2 = m > 0 ? sS[a * a + a - m - k] : cplx{0, 0};
+          const double sm = (m & 1) ? -1.0 : 1.0, sj = (j & 1) ? -1.0 : 1.0;
+          const int pr = row & 1, pc = col & 1;
+          if (pr == 0 && pc == 0) v = sm * S1.re + S2.re;
+          if (pr == 0 && pc == 1) v = -sm * S1.im + S2.im;
+          if (pr == 1 && pc == 0) v = sm * S1.im + S2.im;
+          if (pr == 1 && pc == 1) v = sm * S1.re - S2.re;
+          v *= sj;
+        }
+      }
+      T[((size_t)o * S::NBLK + S::base(rb) + kc) * 64 + lane] = (R)v;
+    }
+}
+
+// Target tiles: cells with M2L pairs keyed by (level, octant); a radix sort
+// on the key keeps cell order inside a group.
+__global__ void k_tile_key(int nc, const int *__restrict__ off, const int *__restrict__ lvl,
+                           const uint64_t *__restrict__ ckey, int *__restrict__ key, int *__restrict__ val) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int k = 255;
+  const int l = lvl[c];
+  if (off[c + 1] > off[c] && l > 0 && l < 31) {
+    const uint64_t lvloff = ((1ull << (3 * l)) - 1) / 7;
+    k = l * 8 + (int)((ckey[c] - lvloff) & 7);
+  }
+  key[c] = k;
+  val[c] = c;
+}
+
+__global__ void k_tile_hist(int nc, const int *__restrict__ key, int *__restrict__ hist) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc && key[c] < 255) atomicAdd(&hist[key[c]], 1);
+}
+
+// gb[0..255] = group start in sorted order, gb[256..511] = first tile of the group.
+__global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *__restrict__ sval,
+                              const int *__restrict__ gb, const double *__restrict__ cRb, int *__restrict__ tile_tgt,
+                              int *__restrict__ tile_of, int *__restrict__ col_of, double *__restrict__ tile_u) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nc) return;
+  const int g = skey[p], c = sval[p];
+  if (g >= 255) {
+    tile_of[c] = -1;
+    return;
+  }
+  const int r = p - gb[g];
+  const int t = gb[256 + g] + r / 64, col = r % 64;
+  tile_tgt[t * 64 + col] = c;
+  tile_of[c] = t;
+  col_of[c] = col;
+  if (col == 0) tile_u[t] = 2 * cRb[c] / kLatUnit;
+}
+
+__device__ inline int pair_target(const int *__restrict__ off, int nc, int64_t e) {
+  int lo = 0, hi = nc;  // largest c with off[c] <= e
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Mark the (tile, offset) steps that have at least one pair.
+__global__ void k_lat_mark(const int *__restrict__ off, int nc, int64_t np, const int *__restrict__ pkey,
+                           const int *__restrict__ o_of_key, const int *__restrict__ tile_of, int noff,
+                           int *__restrict__ act) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  const int key = pkey[e];
+  const int o = key >= 0 ? o_of_key[key] : -1;
+  if (o < 0) return;
+  const int tile = tile_of[pair_target(off, nc, e)];
+  if (tile >= 0) act[(size_t)tile * noff + o] = 1;
+}
+
+// Compact step list: alist[k] = offset of step k (steps of a tile in offset order).
+__global__ void k_lat_alist(const int *__restrict__ act, const int *__restrict__ aidx, int64_t n, int noff,
+                            int *__restrict__ alist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && act[i]) alist[aidx[i]] = (int)(i % noff);
+}
+
+// Lattice pairs: their column in their step's slot row; the rest: irregular.
+__global__ void k_lat_slots(const int *__restrict__ off, int nc, const int *__restrict__ src, int64_t np,
+                            const int *__restrict__ pkey, const int *__restrict__ o_of_key,
+                            const int *__restrict__ tile_of, const int *__restrict__ col_of, int noff,
+                            const int *__restrict__ aidx, int *__restrict__ slot, int *__restrict__ irr) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  const int key = pkey[e];
+  const int o = key >= 0 ? o_of_key[key] : -1;
+  const int t = pair_target(off, nc, e);
+  const int tile = tile_of[t];
+  // A second pair at an occupied slot (a repeated source) stays per pair.
+  if (o >= 0 && tile >= 0 &&
+      atomicCAS(&slot[(size_t)aidx[(size_t)tile * noff + o] * 64 + col_of[t]], -1, src[e]) == -1)
+    irr[e] = 0;
+  else
+    irr[e] = 1;
+}
+
+// Offsets used by at least kLatMinPairs pairs, numbered in key order.
+__global__ void k_lat_flag(const int *__restrict__ hist, int n, int *__restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = hist[k] >= kLatMinPairs ? 1 : 0;
+}
+
+__global__ void k_lat_number(const int *__restrict__ flag, const int *__restrict__ scan, int n,
+                             int *__restrict__ o_of_key, int *__restrict__ okey) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  o_of_key[k] = flag[k] ? scan[k] : -1;
+  if (flag[k]) okey[scan[k]] = k;
+}
+
+__global__ void k_flag_nonzero_off(const int *__restrict__ off, int nc, int *__restrict__ flag) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= nc) flag[c] = (c < nc && off[c + 1] > off[c]) ? 1 : 0;
+}
+
+__global__ void k_compact_idx(const int *__restrict__ flag, const int *__restrict__ scan, int n,
+                              int *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n && flag[c]) out[scan[c]] = c;
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// CSR of the irregular pairs: offsets from the exclusive scan of the flags.
+__global__ void k_irr_csr(const int *__restrict__ off, int nc, const int *__restrict__ scan,
+                          int *__restrict__ off_irr) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= nc) off_irr[c] = scan[off[c]];
+}
+
+__global__ void k_irr_compact(const int *__restrict__ irr, const int *__restrict__ scan, int64_t np,
+                              const int *__restrict__ src, const unsigned char *__restrict__ img,
+                              int *__restrict__ src_irr, unsigned char *__restrict__ img_irr) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np || !irr[e]) return;
+  src_irr[scan[e]] = src[e];
+  img_irr[scan[e]] = img[e];
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup (4 waves) per (tile of 64 target cells, segment of the
+// tile's steps).  R = double: FP64 MFMA.  R = float (mixed form): FP32
+// tables and MFMA, the FP32 accumulators flushed into FP64 every kFlush
+// steps.  T of the next step is loaded into registers while the current
+// step runs and stored into the other LDS buffer (one barrier per step).
+// The segment's partial expansions (unscaled) go to part[] for k_lat_reduce.
+constexpr int kFlush = 16;
+
+template <int P, typename R>
+__global__ void __launch_bounds__(256)
+    k_m2l_mfma(int nseg, const int *__restrict__ step0, const int *__restrict__ alist, const double *__restrict__ tile_u,
+               const int *__restrict__ slot, const R *__restrict__ T, const cplx *__restrict__ M,
+               double *__restrict__ part) {
+  using S = MfmaShape<P>;
+  using A = MfmaAcc<R>;
+  constexpr bool F32 = sizeof(R) == 4;
+  constexpr int NR = S::NR, RB = S::RB, KC = S::KC, NBLK = S::NBLK;
+  constexpr int TE = NBLK * 64;               // T entries per step
+  constexpr int TPT = (TE + 255) / 256;       // T entries per thread
+  __shared__ R sT[2][TE];
+  __shared__ double sPw[P + 1];  // u^-n
+  constexpr DegreeTable<P> deg{};
+  const int tile = blockIdx.x / nseg, seg = blockIdx.x % nseg;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = wave * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  const int a0 = step0[tile], a1 = step0[tile + 1];
+  const int i0 = a0 + (int)((int64_t)(a1 - a0) * seg / nseg), i1 = a0 + (int)((int64_t)(a1 - a0) * (seg + 1) / nseg);
+  const double *Mr = reinterpret_cast<const double *>(M);
+  if (tid == 0) {
+    const double iw = 1.0 / tile_u[tile];
+    double f = 1;
+    for (int i = 0; i <= P; i++) {
+      sPw[i] = f;
+      f *= iw;
+    }
+  }
+  if (i0 < i1) {
+    const R *To = T + (size_t)alist[i0] * TE;
+    for (int i = tid; i < TE; i += 256) sT[0][i] = To[i];
+  }
+  __syncthreads();
+  typename A::type acc[RB];
+  v4d acc64[F32 ? RB : 1];
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++) acc[rb] = typename A::type{0, 0, 0, 0};
+#pragma unroll
+  for (int rb = 0; rb < (F32 ? RB : 1); rb++) acc64[rb] = v4d{0, 0, 0, 0};
+  int active = 0;
+  for (int i = i0; i < i1; i++) {
+    const int cur = (i - i0) & 1;
+    // Next step's T into registers.
+    R tn[TPT];
+    if (i + 1 < i1) {
+      const R *Tn = T + (size_t)alist[i + 1] * TE;
+#pragma unroll
+      for (int q = 0; q < TPT; q++) {
+        const int idx = tid + 256 * q;
+        tn[q] = idx < TE ? Tn[idx] : (R)0;
+      }
+    }
+    const int s = slot[(size_t)i * 64 + col];
+    R b[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+      // Row k = 4 kc + kq of B: coefficient t = k / 2 = 2 kc + kq / 2.
+      const int k = kc * 4 + kq;
+      const int n = (kq < 2) ? deg.n[2 * kc] : deg.n[2 * kc + 1 < S::NT ? 2 * kc + 1 : S::NT];
+      b[kc] = (R)((s >= 0 && k < NR) ? Mr[(size_t)s * NR + k] * sPw[n] : 0.0);
+    }
+    const R *sTc = sT[cur];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+      for (int kc = 0; kc < S::kc_end(rb); kc++) acc[rb] = A::mac(sTc[(S::base(rb) + kc) * 64 + lane], b[kc], acc[rb]);
+    }
+    if (F32 && ++active == kFlush) {
+      active = 0;
+#pragma unroll
+      for (int rb = 0; rb < (F32 ? RB : 1); rb++)
+        for (int r = 0; r < 4; r++) {
+          acc64[rb][r] += (double)acc[rb][r];
+          acc[rb][r] = 0;
+        }
+    }
+    if (i + 1 < i1) {
+#pragma unroll
+      for (int q = 0; q < TPT; q++) {
+        const int idx = tid + 256 * q;
+        if (idx < TE) sT[cur ^ 1][idx] = tn[q];
+      }
+    }
+    __syncthreads();
+  }
+  double *pp = part + (size_t)blockIdx.x * (RB * 16) * 64;
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = rb * 16 + d_row<R>(lane, r);
+      const double v = F32 ? acc64[F32 ? rb : 0][r] + (double)acc[rb][r] : (double)acc[rb][r];
+      pp[(size_t)row * 64 + col] = v;
+    }
+}
+
+// L of each tile column += u^-(j+1) * (sum of the segments' partials, in
+// segment order).
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_lat_reduce(int nseg, const int *__restrict__ tile_tgt, const double *__restrict__ tile_u,
+                 const double *__restrict__ part, cplx *__restrict__ L) {
+  using S = MfmaShape<P>;
+  constexpr int NR = S::NR, NRP = S::RB * 16;
+  constexpr DegreeTable<P> deg{};
+  const int tile = blockIdx.x;
+  const double iw = 1.0 / tile_u[tile];
+  double *Lr = reinterpret_cast<double *>(L);
+  for (int i = threadIdx.x; i < NR * 64; i += 256) {
+    const int row = i / 64, col = i % 64;
+    const int tgt = tile_tgt[tile * 64 + col];
+    if (tgt < 0) continue;
+    double v = 0;
+    for (int sg = 0; sg < nseg; sg++) v += part[((size_t)(tile * nseg + sg) * NRP + row) * 64 + col];
+    double f = iw;
+    for (int q = 0; q < deg.n[row >> 1]; q++) f *= iw;
+    Lr[(size_t)tgt * NR + row] += v * f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int P>
+static void lattice_run(Handle &h, M2LList &irr) {
+  using S = MfmaShape<P>;
+  const int nc = h.ncells;
+  const int64_t np = h.nm2l;
+  const int TPB = 256;
+  const int gp = div_up(np, TPB);
+  // 1. Classify pairs and histogram their offsets.
+  h.lat_pkey.resize(np > 0 ? np : 1);
+  h.lat_hist.resize(kLatKeys);
+  hipLaunchKernelGGL(k_fill_int, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, (int64_t)kLatKeys,
+                     0);
+  hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.m2l_src.p, h.m2l_img.p, np,
+                     h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_hist.p);
+  // 2. Offsets with enough pairs, numbered in key order.
+  h.lat_flag.resize(kLatKeys + 1);
+  h.lat_kscan.resize(kLatKeys + 1);
+  h.lat_ookey.resize(kLatKeys);
+  h.lat_okey.resize(kLatKeys);
+  hipLaunchKernelGGL(k_lat_flag, dim3(div_up(kLatKeys + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, kLatKeys,
+                     h.lat_flag.p);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_flag.p + kLatKeys, (int64_t)1, 0);
+  exclusive_scan_int(h, h.lat_flag.p, h.lat_kscan.p, kLatKeys + 1);
+  hipLaunchKernelGGL(k_lat_number, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_flag.p, h.lat_kscan.p,
+                     kLatKeys, h.lat_ookey.p, h.lat_okey.p);
+  // 3. Target tiles of one level and one octant (cell order inside a tile).
+  h.lat_tkey.resize(2 * (size_t)nc);
+  h.lat_tval.resize(2 * (size_t)nc);
+  h.lat_thist.resize(256);
+  int *tk = h.lat_tkey.p, *tv = h.lat_tval.p;
+  hipLaunchKernelGGL(k_tile_key, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, h.m2l_off.p, h.cLevel.p, h.cKey.p,
+                     tk, tv);
+  bool alt = false;
+  sort_pairs_int_int(h, tk, tk + nc, tv, tv + nc, nc, 8, alt);
+  const int *skey = alt ? tk + nc : tk, *sval = alt ? tv + nc : tv;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, h.stream, h.lat_thist.p, (int64_t)256, 0);
+  hipLaunchKernelGGL(k_tile_hist, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, h.lat_thist.p);
+  int ghist[256], noff = 0;
+  EXA_HIP(hipMemcpyAsync(ghist, h.lat_thist.p, sizeof(ghist), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(&noff, h.lat_kscan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  int gb[512], ntiles = 0, start = 0;
+  for (int g = 0; g < 256; g++) {
+    gb[g] = start;
+    gb[256 + g] = ntiles;
+    start += g < 255 ? ghist[g] : 0;
+    ntiles += g < 255 ? div_up(ghist[g], 64) : 0;
+  }
+  h.lat_gb.resize(512);
+  EXA_HIP(hipMemcpyAsync(h.lat_gb.p, gb, sizeof(gb), hipMemcpyHostToDevice, h.stream));
+  h.lat_tile_tgt.resize((size_t)(ntiles > 0 ? ntiles : 1) * 64);
+  h.lat_tile_w.resize(ntiles > 0 ? ntiles : 1);
+  h.lat_tile_of.resize(nc);
+  h.lat_col_of.resize(nc);
+  if (ntiles)
+    hipLaunchKernelGGL(k_fill_int, dim3(div_up((int64_t)ntiles * 64, TPB)), dim3(TPB), 0, h.stream, h.lat_tile_tgt.p,
+                       (int64_t)ntiles * 64, -1);
+  hipLaunchKernelGGL(k_tile_assign, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, sval, h.lat_gb.p,
+                     h.cRb.p, h.lat_tile_tgt.p, h.lat_tile_of.p, h.lat_col_of.p, h.lat_tile_w.p);
+  // 4. Steps: (tile, offset) with at least one pair, compacted per tile.
+  const int64_t nsteps_all = (int64_t)ntiles * noff;
+  h.lat_act.resize(nsteps_all + 1);
+  h.lat_aidx.resize(nsteps_all + 1);
+  hipLaunchKernelGGL(k_fill_int, dim3(div_up(nsteps_all + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p,
+                     nsteps_all + 1, 0);
+  if (nsteps_all)
+    hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, np, h.lat_pkey.p,
+                       h.lat_ookey.p, h.lat_tile_of.p, noff, h.lat_act.p);
+  exclusive_scan_int(h, h.lat_act.p, h.lat_aidx.p, (int)(nsteps_all + 1));
+  int nact = 0;
+  EXA_HIP(hipMemcpyAsync(&nact, h.lat_aidx.p + nsteps_all, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  h.lat_alist.resize(nact > 0 ? nact : 1);
+  h.lat_step0.resize(ntiles + 1);
+  if (nsteps_all) {
+    hipLaunchKernelGGL(k_lat_alist, dim3(div_up(nsteps_all, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p, h.lat_aidx.p,
+                       nsteps_all, noff, h.lat_alist.p);
+    // step0[t] = aidx[t * noff]: a strided copy.
+    EXA_HIP(hipMemcpy2DAsync(h.lat_step0.p, sizeof(int), h.lat_aidx.p, (size_t)noff * sizeof(int), sizeof(int),
+                             ntiles + 1, hipMemcpyDeviceToDevice, h.stream));
+  }
+  const int64_t nslot = (int64_t)nact * 64;
+  h.lat_slot.resize(nslot > 0 ? nslot : 1);
+  if (nslot)
+    hipLaunchKernelGGL(k_fill_int, dim3(div_up(nslot, TPB)), dim3(TPB), 0, h.stream, h.lat_slot.p, nslot, -1);
+  h.lat_irr.resize(np + 1);
+  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.m2l_src.p, np, h.lat_pkey.p,
+                     h.lat_ookey.p, h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p, h.lat_irr.p);
+  // 5. The irregular pairs' CSR.
+  h.lat_scan.resize(np + 1);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_irr.p + np, (int64_t)1, 0);
+  exclusive_scan_int(h, h.lat_irr.p, h.lat_scan.p, (int)(np + 1));
+  h.irr_off.resize(nc + 1);
+  hipLaunchKernelGGL(k_irr_csr, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.lat_scan.p,
+                     h.irr_off.p);
+  int nirr = 0;
+  EXA_HIP(hipMemcpyAsync(&nirr, h.lat_scan.p + np, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  int nitl = 0;
+  if (nirr) {
+    h.irr_src.resize(nirr);
+    h.irr_img.resize(nirr);
+    hipLaunchKernelGGL(k_irr_compact, dim3(gp), dim3(TPB), 0, h.stream, h.lat_irr.p, h.lat_scan.p, np, h.m2l_src.p,
+                       h.m2l_img.p, h.irr_src.p, h.irr_img.p);
+    // Targets with irregular pairs.
+    h.ie.reserve(2 * (size_t)nc + 2);
+    int *flag = h.ie.p, *scan = h.ie.p + nc + 1;
+    hipLaunchKernelGGL(k_flag_nonzero_off, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.irr_off.p, nc, flag);
+    exclusive_scan_int(h, flag, scan, nc + 1);
+    h.irr_tgt.resize(nc);
+    hipLaunchKernelGGL(k_compact_idx, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, flag, scan, nc, h.irr_tgt.p);
+    EXA_HIP(hipMemcpyAsync(&nitl, scan + nc, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+  }
+  irr = M2LList{h.irr_tgt.p, nitl, h.irr_off.p, h.irr_src.p, h.irr_img.p};
+  h.lat_stats_pairs = np - nirr;
+  h.lat_stats_noff = noff;
+  h.lat_stats_tiles = ntiles;
+  // 6. Matrices, the contraction and the reduction.
+  if (noff == 0 || ntiles == 0 || nact == 0) return;
+  const int nseg = std::max(1, std::min(8, div_up(2048, ntiles)));
+  h.lat_part.resize((size_t)ntiles * nseg * S::RB * 16 * 64);
+  EXA_HIP(hipEventRecord(h.ev[12], h.stream));
+  h.lat_timed = true;
+  if (h.cfg.precision == 1) {
+    h.lat_Tf.resize((size_t)noff * S::NBLK * 64);
+    hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_Tf.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, float>), dim3(ntiles * nseg), dim3(256), 0, h.stream, nseg, h.lat_step0.p,
+                       h.lat_alist.p, h.lat_tile_w.p, h.lat_slot.p, h.lat_Tf.p, h.M.p, h.lat_part.p);
+  } else {
+    h.lat_T.resize((size_t)noff * S::NBLK * 64);
+    hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_T.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, double>), dim3(ntiles * nseg), dim3(256), 0, h.stream, nseg, h.lat_step0.p,
+                       h.lat_alist.p, h.lat_tile_w.p, h.lat_slot.p, h.lat_T.p, h.M.p, h.lat_part.p);
+  }
+  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, nseg, h.lat_tile_tgt.p, h.lat_tile_w.p,
+                     h.lat_part.p, h.L.p);
+  EXA_HIP(hipEventRecord(h.ev[13], h.stream));
+  EXA_HIP(hipGetLastError());
+}
+
+bool run_m2l_lattice(Handle &h, M2LList &irr) {
+  h.lat_timed = false;
+  h.lat_stats_pairs = 0;
+  h.lat_stats_noff = h.lat_stats_tiles = 0;
+  if (h.cfg.basis != 0 || h.cfg.mass || h.nm2l == 0 || !h.lattice_m2l) return false;
+  switch (h.cfg.P) {
+    case 4: lattice_run<4>(h, irr); return true;
+    case 6: lattice_run<6>(h, irr); return true;
+    case 8: lattice_run<8>(h, irr); return true;
+    case 10: lattice_run<10>(h, irr); return true;
+    case 12: lattice_run<12>(h, irr); return true;
+    default: return false;
+  }
+}
+
+}  // namespace exafmm

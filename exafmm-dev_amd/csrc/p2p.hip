@@ -25,12 +25,91 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
 
 constexpr int kP2PStage = 128;  // source bodies staged in LDS per fill
 
+// Walker over a target's source list (CSR entries [e, e1)).  The
+// descriptors of up to 64 consecutive entries -- first body, body count,
+// image -- and the exclusive prefix of their counts are loaded in one
+// parallel step; a stage fill then maps each stage slot to (entry, body) by
+// binary search over that prefix, so all body loads of a fill are
+// independent of each other (no per-leaf dependent load chain).  The
+// workgroup is one wave; every call is uniform across it.
+struct SrcWalker {
+  int e, sbase, dbase, dcount;
+};
+
+struct SrcDesc {
+  int b0[64];
+  int pre[65];
+  unsigned char img[64];
+};
+
+template <int STAGE, typename Put>
+__device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__restrict__ src,
+                                 const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                                 const int *__restrict__ cNBody, SrcDesc &sd, Put put) {
+  __syncthreads();  // the previous stage (and descriptor batch) are consumed
+  if (w.e >= w.dbase + w.dcount) {
+    w.dbase = w.e;
+    w.dcount = min(64, e1 - w.e);
+    int n = 0;
+    if (lane < w.dcount) {
+      const int s = src[w.e + lane];
+      sd.b0[lane] = cBody[s];
+      n = cNBody[s];
+      sd.img[lane] = img[w.e + lane];
+    }
+    int incl = n;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    sd.pre[lane] = incl - n;
+    if (lane == 63) sd.pre[64] = incl;
+    __syncthreads();
+  }
+  const int first = w.e - w.dbase;
+  const int q0 = sd.pre[first] + w.sbase;
+  const int qend = sd.pre[64];
+  const int cnt = min(STAGE, qend - q0);
+  for (int k = lane; k < cnt; k += 64) {
+    const int q = q0 + k;
+    int lo = first, hi = w.dcount;  // largest i in [first, dcount) with pre[i] <= q
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sd.pre[mid] <= q)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    put(k, sd.b0[lo] + (q - sd.pre[lo]), sd.img[lo]);
+  }
+  // Advance to the first unconsumed body.
+  const int qn = q0 + cnt;
+  if (qn >= qend) {
+    w.e = w.dbase + w.dcount;
+    w.sbase = 0;
+  } else {
+    int lo = first, hi = w.dcount;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sd.pre[mid] <= qn)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    w.e = w.dbase + lo;
+    w.sbase = qn - sd.pre[lo];
+  }
+  __syncthreads();  // the stage is filled
+  return cnt;
+}
+
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
               const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
               const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
   __shared__ double4 sb[kP2PStage];
   __shared__ double4 sacc[64];
+  __shared__ SrcDesc sd;
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
   const int tb0 = cBody[c], tn = cNBody[c];
@@ -42,39 +121,26 @@ __global__ void __launch_bounds__(64)
     const bool active = tbase + ti < tn;
     const double4 xi = active ? body[tb0 + tbase + ti] : make_double4(0, 0, 0, 0);
     double pot = 0, ax = 0, ay = 0, az = 0;
-    int e = e0, sbase = 0;
-    while (e < e1) {
+    SrcWalker w{e0, 0, e0, 0};
+    while (w.e < e1) {
       // Fill: consecutive source leaves (or parts of one) up to kP2PStage
       // bodies, each shifted by its periodic image.
-      int cnt = 0;
-      __syncthreads();
-      while (e < e1 && cnt < kP2PStage) {
-        const int s = src[e];
-        const int sb0 = cBody[s], sn = cNBody[s];
-        double px = 0, py = 0, pz = 0;
-        image_shift(img[e], cycle, px, py, pz);
-        const int take = min(sn - sbase, kP2PStage - cnt);
-        for (int k = lane; k < take; k += 64) {
-          double4 v = body[sb0 + sbase + k];
-          v.x += px;
-          v.y += py;
-          v.z += pz;
-          sb[cnt + k] = v;
-        }
-        cnt += take;
-        sbase += take;
-        if (sbase == sn) {
-          e++;
-          sbase = 0;
-        }
-      }
-      __syncthreads();
+      const int cnt = fill_stage<kP2PStage>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
+        double px, py, pz;
+        image_shift(im, cycle, px, py, pz);
+        double4 v = body[b];
+        v.x += px;
+        v.y += py;
+        v.z += pz;
+        sb[k] = v;
+      });
 #pragma unroll 4
       for (int k = g; k < cnt; k += G) {
         const double4 xj = sb[k];
         const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const double invr = r2 != 0 ? rsqrt(r2) : 0.0;
+        const double rs = rsqrt(r2);  // unconditional: a select, not a branch, skips R2 == 0
+        const double invr = r2 > 0 ? rs : 0.0;
         const double qinvr = xj.w * invr;
         const double qinvr3 = qinvr * invr * invr;
         pot += qinvr;
@@ -139,6 +205,7 @@ __global__ void __launch_bounds__(64)
   __shared__ float4 sb[kP2PStageF];   // hi parts and charge
   __shared__ float4 sbl[kP2PStageF];  // lo parts
   __shared__ double4 sacc[64];
+  __shared__ SrcDesc sd;
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
   const int tb0 = cBody[c], tn = cNBody[c];
@@ -154,30 +221,17 @@ __global__ void __launch_bounds__(64)
     const float xi = (float)rx, yi = (float)ry, zi = (float)rz;
     const float xil = (float)(rx - xi), yil = (float)(ry - yi), zil = (float)(rz - zi);
     Kahan32 kp, kx, ky, kz;
-    int e = e0, sbase = 0;
-    while (e < e1) {
-      int cnt = 0;
-      __syncthreads();
-      while (e < e1 && cnt < kP2PStageF) {
-        const int s = src[e];
-        const int sb0 = cBody[s], sn = cNBody[s];
-        double px = 0, py = 0, pz = 0;
-        image_shift(img[e], cycle, px, py, pz);
-        const int take = min(sn - sbase, kP2PStageF - cnt);
-        for (int k = lane; k < take; k += 64) {
-          const double4 v = body[sb0 + sbase + k];
-          const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
-          const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-          sb[cnt + k] = make_float4(hx, hy, hz, (float)v.w);
-          sbl[cnt + k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
-        }
-        cnt += take;
-        sbase += take;
-        if (sbase == sn) {
-          e++;
-          sbase = 0;
-        }
-      }
+    SrcWalker w{e0, 0, e0, 0};
+    while (w.e < e1) {
+      const int cnt = fill_stage<kP2PStageF>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
+        double px, py, pz;
+        image_shift(im, cycle, px, py, pz);
+        const double4 v = body[b];
+        const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
+        const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+        sb[k] = make_float4(hx, hy, hz, (float)v.w);
+        sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
+      });
       // Pad the tile to whole segments with zero charges (they add exactly 0).
       const int cntp = (cnt + kSeg * G - 1) / (kSeg * G) * (kSeg * G);
       for (int k = cnt + lane; k < cntp; k += 64) {
@@ -195,7 +249,8 @@ __global__ void __launch_bounds__(64)
           const float dx = (xi - xj.x) + (xil - xl.x), dy = (yi - xj.y) + (yil - xl.y),
                       dz = (zi - xj.z) + (zil - xl.z);
           const float r2 = dx * dx + dy * dy + dz * dz;
-          const float invr = r2 != 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+          const float rs = __builtin_amdgcn_rsqf(r2);
+          const float invr = r2 > 0.0f ? rs : 0.0f;
           const float qinvr = xj.w * invr;
           const float qinvr3 = qinvr * invr * invr;
           pot += qinvr;

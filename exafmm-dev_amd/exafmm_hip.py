@@ -51,7 +51,9 @@ class Stats(ctypes.Structure):
                 ("t_bounds", "t_build", "t_upward", "t_lists", "t_p2p", "t_m2l", "t_downward", "t_total",
                  "t_traverse")] + [
         ("num_p2p_pairs", ctypes.c_int64), ("num_m2l_pairs", ctypes.c_int64), ("num_p2p_bodies", ctypes.c_int64),
-        ("num_cells", ctypes.c_int), ("num_levels", ctypes.c_int)]
+        ("num_cells", ctypes.c_int), ("num_levels", ctypes.c_int), ("num_m2l_lattice", ctypes.c_int64),
+        ("num_m2l_offsets", ctypes.c_int), ("num_m2l_tiles", ctypes.c_int), ("t_m2l_mfma", ctypes.c_double),
+        ("t_m2l_pairs", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

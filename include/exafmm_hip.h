@@ -63,6 +63,12 @@ typedef struct {
   int64_t num_p2p_bodies;  /* target-source body pairs evaluated by P2P       */
   int num_cells;
   int num_levels;
+  int64_t num_m2l_lattice; /* M2L pairs evaluated as offset-grouped MFMA
+                              contractions (the rest per pair)                */
+  int num_m2l_offsets;     /* distinct lattice offsets (one matrix each)       */
+  int num_m2l_tiles;       /* 64-cell target tiles of the contraction          */
+  double t_m2l_mfma;       /* ms of the MFMA contraction kernel alone           */
+  double t_m2l_pairs;      /* ms of the per-pair M2L kernel alone               */
 } exafmm_hip_stats;
 
 /* Create / destroy.  Replaces the constructors BoundBox(nspawn),

@@ -171,3 +171,24 @@ def test_single_body_and_duplicates():
     otrg[ob["ibody"]] = ob["trg"]
     assert rel_l2(f.results(), otrg) < 1e-12
     f.close()
+
+
+@pytest.mark.parametrize("name,builder", [("cube16k_u_p10", "uniform"), ("plummer16k_a_p10", "adaptive"),
+                                          ("cube4k_a_p12", "adaptive"), ("cube16k_u_p10_mut", "uniform")])
+def test_lattice_m2l_matches_per_pair(name, builder, monkeypatch):
+    """The offset-grouped MFMA contraction gives the per-pair kernel's local
+    expansions (to rounding) and covers the same-level and adjacent-level
+    pairs."""
+    z = load_case(name)
+    out = {}
+    for mode in ("pairs", "lattice"):
+        monkeypatch.setenv("EXAFMM_HIP_M2L", mode)
+        f = gpu_fmm(z, builder)
+        f.evaluate()
+        out[mode] = (f.cells()["L"], f.results(), f.stats())
+        f.close()
+    (La, ra, sa), (Lb, rb, sb) = out["pairs"], out["lattice"]
+    assert sa["num_m2l_lattice"] == 0
+    assert sb["num_m2l_lattice"] > 0.5 * sb["num_m2l_pairs"], (sb["num_m2l_lattice"], sb["num_m2l_pairs"])
+    assert np.abs(Lb - La).max() <= 1e-12 * np.abs(La).max()
+    assert rel_l2(rb, ra) < 1e-13
