@@ -126,8 +126,8 @@ struct Handle {
   DBuf<int> lat_pkey, lat_hist, lat_flag, lat_kscan, lat_okey, lat_ookey, lat_tile_tgt, lat_tile_of, lat_col_of, lat_slot, lat_irr,
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
-  DBuf<double> lat_tile_w, lat_T;
-  DBuf<float> lat_Tf;
+  DBuf<double> lat_tile_w, lat_T, lat_Ms;
+  DBuf<float> lat_Tf, lat_Msf;
   int64_t lat_stats_pairs = 0;
   int lat_stats_noff = 0, lat_stats_tiles = 0;
   bool lat_timed = false, pair_timed = false;  // ev[12..13]: MFMA kernel, ev[14..15]: per-pair kernel

@@ -28,6 +28,7 @@
 // not lattice pairs (levels two or more apart, periodic images, MASS mode,
 // offsets beyond kLatD) go to the per-pair kernel.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "harmonics.h"
@@ -117,8 +118,11 @@ __global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *_
   }
   const int t = lo, s = src[e];
   int key = -1;
-  if (img[e] == 13) {
-    const double u = 2 * cRb[t] / kLatUnit;
+  // Same level or one level apart: then the offset's residue mod 4 fixes
+  // the level relation (0: same, 2: source one up, odd: source one down).
+  const double rt = cRb[t], rs = cRb[s];
+  if (img[e] == 13 && (rs == rt || rs == 2 * rt || 2 * rs == rt)) {
+    const double u = 2 * rt / kLatUnit;
     const double4 a = cX[t], b = cX[s];
     const double dx = (a.x - b.x) / u, dy = (a.y - b.y) / u, dz = (a.z - b.z) / u;
     const double rx = rint(dx), ry = rint(dy), rz = rint(dz);
@@ -132,6 +136,16 @@ __global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *_
   if (key >= 0 && hist[key] == 0) hist[key] = 1;
 }
 
+// T(d) rows in memory and LDS: TE entries padded to whole 1 KiB chunks (one
+// chunk = one wave-wide 16-byte-per-lane LDS DMA).
+template <int P, typename R>
+struct TPad {
+  static constexpr int TE = MfmaShape<P>::NBLK * 64;
+  static constexpr int CH = 1024 / (int)sizeof(R);
+  static constexpr int NCH = (TE + CH - 1) / CH;
+  static constexpr int TEP = NCH * CH;
+};
+
 // T(d) for every used offset, in fragment order: block (rb, kc) holds
 // T[rb*16 + l%16][kc*4 + l/16] at lane l.
 template <int P, typename R>
@@ -144,6 +158,11 @@ __global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R
   const double dx = key / (kLatSide * kLatSide) - kLatD, dy = (key / kLatSide) % kLatSide - kLatD,
                dz = key % kLatSide - kLatD;
   const Geom g = make_geom(dx, dy, dz);
+  // Source multipoles arrive scaled by their own cell's u^-n; u_s = 2^D u_t
+  // with D = 0, +1, -1 for offsets = 0, 2, odd mod 4, so column (n, m) of T
+  // takes the exact factor 2^(D n).
+  const int rx = ((key / (kLatSide * kLatSide)) - kLatD) & 3;
+  const int dlev = rx == 0 ? 0 : (rx == 2 ? 1 : -1);
   for (int t = lane; t < NT; t += 64) {
     int a, b;
     coef_nm(t, a, b);
@@ -172,11 +191,12 @@ __global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R
           if (pr == 0 && pc == 1) v = -sm * S1.im + S2.im;
           if (pr == 1 && pc == 0) v = sm * S1.im + S2.im;
           if (pr == 1 && pc == 1) v = sm * S1.re - S2.re;
-          v *= sj;
+          v = ldexp(v * sj, dlev * n);
         }
       }
-      T[((size_t)o * S::NBLK + S::base(rb) + kc) * 64 + lane] = (R)v;
+      T[(size_t)o * TPad<P, R>::TEP + (S::base(rb) + kc) * 64 + lane] = (R)v;
     }
+  for (int i = S::NBLK * 64 + lane; i < TPad<P, R>::TEP; i += 64) T[(size_t)o * TPad<P, R>::TEP + i] = 0;
 }
 
 // Target tiles: cells with M2L pairs keyed by (level, octant); a radix sort
@@ -322,84 +342,104 @@ __global__ void k_irr_compact(const int *__restrict__ irr, const int *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup (4 waves) per (tile of 64 target cells, segment of the
-// tile's steps).  R = double: FP64 MFMA.  R = float (mixed form): FP32
-// tables and MFMA, the FP32 accumulators flushed into FP64 after every
-// step (kFlush = 1: a step's K-sum is the only FP32 accumulation).  T of the next step is loaded into registers while the current
-// step runs and stored into the other LDS buffer (one barrier per step).
-// The segment's partial expansions (unscaled) go to part[] for k_lat_reduce.
-constexpr int kFlush = 1;
-
+// Source multipoles scaled by their own cell's u^-n (u = w / kLatUnit), in
+// the precision of the contraction, KC * 4 entries per cell (zero padded);
+// row nc is all zero and stands for an empty column.
 template <int P, typename R>
 __global__ void __launch_bounds__(256)
-    k_m2l_mfma(int nseg, const int *__restrict__ step0, const int *__restrict__ alist, const double *__restrict__ tile_u,
-               const int *__restrict__ slot, const R *__restrict__ T, const cplx *__restrict__ M,
-               double *__restrict__ part) {
+    k_lat_mscale(int nc, const double *__restrict__ cRb, const cplx *__restrict__ M, R *__restrict__ Ms) {
+  using S = MfmaShape<P>;
+  constexpr int NR = S::NR, NK = S::KC * 4;
+  constexpr DegreeTable<P> deg{};
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)(nc + 1) * NK) return;
+  const int c = (int)(i / NK), k = (int)(i % NK);
+  double v = 0;
+  if (c < nc && k < NR) {
+    const double iu = kLatUnit / (2 * cRb[c]);
+    double f = 1;
+    for (int q = 0; q < deg.n[k >> 1]; q++) f *= iu;
+    v = reinterpret_cast<const double *>(M)[(size_t)c * NR + k] * f;
+  }
+  Ms[i] = (R)v;
+}
+
+// Steps of a tile are split into kLatSeg segments; segment s runs on XCD s
+// (workgroup w = 8 tile + s, and workgroups are dealt round-robin over the
+// 8 XCDs), so an XCD only ever needs the T(d) of one eighth of the offsets
+// and keeps them in its L2.
+constexpr int kLatSeg = 8;
+
+typedef __attribute__((address_space(3))) void lds_void;
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left alone.
+constexpr int kVmcnt0 = 0x0F70;
+typedef __attribute__((address_space(1))) void glb_void;
+
+// One workgroup (4 waves) per (tile of 64 target cells, segment of the
+// tile's steps).  R = double: FP64 MFMA.  R = float (mixed form): FP32
+// tables, multipoles and MFMA, the FP32 accumulators flushed into FP64 after
+// every step (a step's K-sum is the only FP32 accumulation).  Software
+// pipeline: while step i runs on the matrix cores, T of step i + 1 streams
+// into the other LDS buffer by LDS DMA (global_load_lds_dwordx4), the
+// multipole columns of step i + 1 stream into registers and the slot indices
+// of step i + 2 are read; one barrier per step.  The two LDS buffers are
+// separate arrays and the step loop is unrolled by two, so every LDS access
+// names its buffer statically.  The segment's partial expansions (unscaled)
+// go to part[] for k_lat_reduce.
+template <int P, typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 10 ? 2 : 1, 2)))
+    k_m2l_mfma(int nc, const int *__restrict__ step0, const int *__restrict__ alist, const int *__restrict__ slot,
+               const R *__restrict__ T, const R *__restrict__ Ms, double *__restrict__ part) {
   using S = MfmaShape<P>;
   using A = MfmaAcc<R>;
+  using TP = TPad<P, R>;
   constexpr bool F32 = sizeof(R) == 4;
-  constexpr int NR = S::NR, RB = S::RB, KC = S::KC, NBLK = S::NBLK;
-  constexpr int TE = NBLK * 64;               // T entries per step
-  constexpr int TPT = (TE + 255) / 256;       // T entries per thread
-  __shared__ R sT[2][TE];
-  __shared__ double sPw[P + 1];  // u^-n
-  constexpr DegreeTable<P> deg{};
-  const int tile = blockIdx.x / nseg, seg = blockIdx.x % nseg;
+  constexpr int RB = S::RB, KC = S::KC, NK = KC * 4;
+  constexpr int TEP = TP::TEP, CH = TP::CH, NCH = TP::NCH, VL = 16 / (int)sizeof(R);
+  __shared__ __attribute__((aligned(16))) R sT0[TEP];
+  __shared__ __attribute__((aligned(16))) R sT1[TEP];
+  const int tile = blockIdx.x >> 3, seg = blockIdx.x & 7;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = wave * 16 + (lane & 15);
   const int kq = lane >> 4;
   const int a0 = step0[tile], a1 = step0[tile + 1];
-  const int i0 = a0 + (int)((int64_t)(a1 - a0) * seg / nseg), i1 = a0 + (int)((int64_t)(a1 - a0) * (seg + 1) / nseg);
-  const double *Mr = reinterpret_cast<const double *>(M);
-  if (tid == 0) {
-    const double iw = 1.0 / tile_u[tile];
-    double f = 1;
-    for (int i = 0; i <= P; i++) {
-      sPw[i] = f;
-      f *= iw;
-    }
-  }
-  if (i0 < i1) {
-    const R *To = T + (size_t)alist[i0] * TE;
-    for (int i = tid; i < TE; i += 256) sT[0][i] = To[i];
-  }
-  __syncthreads();
+  const int i0 = a0 + (a1 - a0) * seg / kLatSeg, i1 = a0 + (a1 - a0) * (seg + 1) / kLatSeg;
   typename A::type acc[RB];
   v4d acc64[F32 ? RB : 1];
 #pragma unroll
   for (int rb = 0; rb < RB; rb++) acc[rb] = typename A::type{0, 0, 0, 0};
 #pragma unroll
   for (int rb = 0; rb < (F32 ? RB : 1); rb++) acc64[rb] = v4d{0, 0, 0, 0};
-  int active = 0;
-  for (int i = i0; i < i1; i++) {
-    const int cur = (i - i0) & 1;
-    // Next step's T into registers.
-    R tn[TPT];
-    if (i + 1 < i1) {
-      const R *Tn = T + (size_t)alist[i + 1] * TE;
+  // T(d) of step i into an LDS buffer: chunk ch by wave ch % 4.
+  auto fetch_T = [&](R *dst, int i) {
+    const R *Tn = T + (size_t)alist[i] * TEP + lane * VL;
+    for (int ch = wave; ch < NCH; ch += 4)
+      __builtin_amdgcn_global_load_lds((glb_void *)(Tn + ch * CH), (lds_void *)(dst + ch * CH), 16, 0, 0);
+  };
+  R b[KC];
+  int sn = -1;
+  // Step i from buffer CUR = (i - i0) & 1 while step i + 1 is fetched into
+  // the other one.
+  auto step = [&](auto curc, int i) {
+    constexpr int CUR = decltype(curc)::value;
+    const R *sTc = CUR ? sT1 : sT0;
+    R *sTn = CUR ? sT0 : sT1;
+    const bool more = i + 1 < i1;
+    R bn[KC];
+    int snn = -1;
+    if (more) {
+      const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
 #pragma unroll
-      for (int q = 0; q < TPT; q++) {
-        const int idx = tid + 256 * q;
-        tn[q] = idx < TE ? Tn[idx] : (R)0;
-      }
+      for (int kc = 0; kc < KC; kc++) bn[kc] = src[kc * 4];
+      fetch_T(sTn, i + 1);
+      if (i + 2 < i1) snn = slot[(size_t)(i + 2) * 64 + col];
     }
-    const int s = slot[(size_t)i * 64 + col];
-    R b[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-      // Row k = 4 kc + kq of B: coefficient t = k / 2 = 2 kc + kq / 2.
-      const int k = kc * 4 + kq;
-      const int n = (kq < 2) ? deg.n[2 * kc] : deg.n[2 * kc + 1 < S::NT ? 2 * kc + 1 : S::NT];
-      b[kc] = (R)((s >= 0 && k < NR) ? Mr[(size_t)s * NR + k] * sPw[n] : 0.0);
-    }
-    const R *sTc = sT[cur];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
       for (int kc = 0; kc < S::kc_end(rb); kc++) acc[rb] = A::mac(sTc[(S::base(rb) + kc) * 64 + lane], b[kc], acc[rb]);
     }
-    if (F32 && ++active == kFlush) {
-      active = 0;
+    if (F32) {
 #pragma unroll
       for (int rb = 0; rb < (F32 ? RB : 1); rb++)
         for (int r = 0; r < 4; r++) {
@@ -407,14 +447,28 @@ __global__ void __launch_bounds__(256)
           acc[rb][r] = 0;
         }
     }
-    if (i + 1 < i1) {
+    if (more) {
 #pragma unroll
-      for (int q = 0; q < TPT; q++) {
-        const int idx = tid + 256 * q;
-        if (idx < TE) sT[cur ^ 1][idx] = tn[q];
-      }
+      for (int kc = 0; kc < KC; kc++) b[kc] = bn[kc];
+      sn = snn;
     }
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);  // this wave's DMA and loads have landed
     __syncthreads();
+  };
+  // Prologue: step i0's T and columns.
+  if (i0 < i1) {
+    fetch_T(sT0, i0);
+    sn = slot[(size_t)i0 * 64 + col];
+    const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) b[kc] = src[kc * 4];
+    sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
+  }
+  __builtin_amdgcn_s_waitcnt(kVmcnt0);
+  __syncthreads();
+  for (int i = i0; i < i1; i += 2) {
+    step(std::integral_constant<int, 0>{}, i);
+    if (i + 1 < i1) step(std::integral_constant<int, 1>{}, i + 1);
   }
   double *pp = part + (size_t)blockIdx.x * (RB * 16) * 64;
 #pragma unroll
@@ -422,7 +476,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = rb * 16 + d_row<R>(lane, r);
-      const double v = F32 ? acc64[F32 ? rb : 0][r] + (double)acc[rb][r] : (double)acc[rb][r];
+      const double v = F32 ? acc64[F32 ? rb : 0][r] : (double)acc[rb][r];
       pp[(size_t)row * 64 + col] = v;
     }
 }
@@ -431,7 +485,7 @@ __global__ void __launch_bounds__(256)
 // segment order).
 template <int P>
 __global__ void __launch_bounds__(256)
-    k_lat_reduce(int nseg, const int *__restrict__ tile_tgt, const double *__restrict__ tile_u,
+    k_lat_reduce(const int *__restrict__ tile_tgt, const double *__restrict__ tile_u,
                  const double *__restrict__ part, cplx *__restrict__ L) {
   using S = MfmaShape<P>;
   constexpr int NR = S::NR, NRP = S::RB * 16;
@@ -444,7 +498,7 @@ __global__ void __launch_bounds__(256)
     const int tgt = tile_tgt[tile * 64 + col];
     if (tgt < 0) continue;
     double v = 0;
-    for (int sg = 0; sg < nseg; sg++) v += part[((size_t)(tile * nseg + sg) * NRP + row) * 64 + col];
+    for (int sg = 0; sg < kLatSeg; sg++) v += part[((size_t)(tile * kLatSeg + sg) * NRP + row) * 64 + col];
     double f = iw;
     for (int q = 0; q < deg.n[row >> 1]; q++) f *= iw;
     Lr[(size_t)tgt * NR + row] += v * f;
@@ -572,22 +626,29 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_stats_tiles = ntiles;
   // 6. Matrices, the contraction and the reduction.
   if (noff == 0 || ntiles == 0 || nact == 0) return;
-  const int nseg = std::max(1, std::min(8, div_up(2048, ntiles)));
-  h.lat_part.resize((size_t)ntiles * nseg * S::RB * 16 * 64);
+  h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
   EXA_HIP(hipEventRecord(h.ev[12], h.stream));
   h.lat_timed = true;
+  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
+  const dim3 grid(ntiles * kLatSeg);
   if (h.cfg.precision == 1) {
-    h.lat_Tf.resize((size_t)noff * S::NBLK * 64);
+    h.lat_Tf.resize((size_t)noff * TPad<P, float>::TEP);
+    h.lat_Msf.resize(nms);
     hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_Tf.p);
-    hipLaunchKernelGGL((k_m2l_mfma<P, float>), dim3(ntiles * nseg), dim3(256), 0, h.stream, nseg, h.lat_step0.p,
-                       h.lat_alist.p, h.lat_tile_w.p, h.lat_slot.p, h.lat_Tf.p, h.M.p, h.lat_part.p);
+    hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
+                       h.lat_Msf.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
+                       h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
   } else {
-    h.lat_T.resize((size_t)noff * S::NBLK * 64);
+    h.lat_T.resize((size_t)noff * TPad<P, double>::TEP);
+    h.lat_Ms.resize(nms);
     hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_T.p);
-    hipLaunchKernelGGL((k_m2l_mfma<P, double>), dim3(ntiles * nseg), dim3(256), 0, h.stream, nseg, h.lat_step0.p,
-                       h.lat_alist.p, h.lat_tile_w.p, h.lat_slot.p, h.lat_T.p, h.M.p, h.lat_part.p);
+    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
+                       h.lat_Ms.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
+                       h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
   }
-  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, nseg, h.lat_tile_tgt.p, h.lat_tile_w.p,
+  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.lat_tile_w.p,
                      h.lat_part.p, h.L.p);
   EXA_HIP(hipEventRecord(h.ev[13], h.stream));
   EXA_HIP(hipGetLastError());
