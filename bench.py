@@ -71,6 +71,17 @@ def cpu_baseline(n, P):
     }
 
 
+def copy_probe():
+    """One device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB): the
+    known-size kernel that scales the HBM byte counters."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    b = a.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 8, "ok": bool(b[-1].item() == 1.0)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +93,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--with-mixed", action="store_true", default=True)
     ap.add_argument("--no-mixed", dest="with_mixed", action="store_false")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; multi-GPU sharding is design-only "
               "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -107,6 +123,7 @@ def main():
             stats.append(f.stats())
         f.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        stats[-1]["num_p2p_bodies"] = f.count_p2p_bodies()  # outside the timed steps
         # Accuracy against FP64 direct summation on 128 random targets.
         res = f.results()[tgt]
         ref = f.direct(tgt)

@@ -105,6 +105,7 @@ static void do_traverse(Handle &h) {
   EXA_HIP(hipStreamWaitEvent(h.stream, h.ev[10], 0));
   EXA_HIP(hipEventRecord(h.ev[7], h.stream));
   h.have_trav = true;
+  h.stats.num_p2p_bodies = -1;  // until count_p2p_bodies
 }
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -421,8 +422,16 @@ int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
     h.stats.num_m2l_lattice = h.lat_stats_pairs;
     h.stats.num_m2l_offsets = h.lat_stats_noff;
     h.stats.num_m2l_tiles = h.lat_stats_tiles;
-    if (h.have_trav) h.stats.num_p2p_bodies = count_p2p_body_pairs(h);
     *s = h.stats;
+  });
+}
+
+int exafmm_hip_count_p2p_bodies(exafmm_hip_handle *H, int64_t *n) {
+  return guarded(H, [&](Handle &h) {
+    ensure(n != nullptr, "count_p2p_bodies: null");
+    ensure(h.have_trav, "count_p2p_bodies: no traversal yet");
+    h.stats.num_p2p_bodies = count_p2p_body_pairs(h);
+    *n = h.stats.num_p2p_bodies;
   });
 }
 

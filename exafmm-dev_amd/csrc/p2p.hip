@@ -7,12 +7,15 @@
 // traversal.h:333-337); the GPU accumulates the normalised quantities
 // directly.
 //
-// Lane layout: a leaf's targets are replicated over G = 64 / T lane groups
-// (T = 16, 32 or 64 targets per group) and group g takes every G-th source
-// body, so short leaves still fill the wavefront.  Source bodies are staged
-// through LDS up to 128 at a time (consecutive list entries, each shifted by
-// its periodic image); the G partial sums are combined in a fixed order at
-// the end (deterministic, no atomics).
+// Lane layout: a leaf's targets (64 at a time) are split into two
+// power-of-two passes, TA = the largest power of two <= n (at least 8) and
+// TB = the remainder rounded up to a power of two (at least 8); in a pass of
+// T targets each target is replicated over G = 64 / T lane groups and group
+// g takes every G-th source body.  Both passes run over every staged source
+// tile, so 40 targets cost 32 + 8 lanes' worth of work instead of 64.
+// Source bodies are staged through LDS (consecutive list entries, each
+// shifted by its periodic image); the G partial sums of a target are
+// combined in a fixed order at the end (deterministic, no atomics).
 #include "internal.h"
 
 namespace exafmm {
@@ -113,6 +116,18 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
   return cnt;
 }
 
+// The two target passes of a block of n <= 64 targets (see above).
+struct TargetPasses {
+  int TA, TB, r;  // r = targets of pass B (0: no pass B)
+  __device__ explicit TargetPasses(int n) {
+    TA = 8;
+    while (TA * 2 <= n) TA *= 2;
+    r = n > TA ? n - TA : 0;
+    TB = 8;
+    while (TB < r) TB *= 2;
+  }
+};
+
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
               const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
@@ -123,14 +138,49 @@ __global__ void __launch_bounds__(64)
   const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
   const int tb0 = cBody[c], tn = cNBody[c];
-  const int T = tn <= 16 ? 16 : (tn <= 32 ? 32 : 64);
-  const int G = 64 / T;
-  const int g = lane / T, ti = lane % T;
   const int e0 = off[c], e1 = off[c + 1];
-  for (int tbase = 0; tbase < tn; tbase += T) {
-    const bool active = tbase + ti < tn;
-    const double4 xi = active ? body[tb0 + tbase + ti] : make_double4(0, 0, 0, 0);
-    double pot = 0, ax = 0, ay = 0, az = 0;
+  auto pair = [](const double4 &xi, const double4 &xj, double4 &acc) {
+    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double rs = rsqrt_pos(r2);  // unconditional: a select, not a branch, skips R2 == 0
+    const double invr = r2 > 0 ? rs : 0.0;
+    const double qinvr = xj.w * invr;
+    const double qinvr3 = qinvr * invr * invr;
+    acc.x += qinvr;
+    acc.y += dx * qinvr3;
+    acc.z += dy * qinvr3;
+    acc.w += dz * qinvr3;
+  };
+  // Sum the G group partials of the pass's targets in group order.
+  auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
+    __syncthreads();
+    sacc[lane] = acc;
+    __syncthreads();
+    if (lane < n) {
+      double4 s = sacc[lane];
+      for (int gg = 1; gg < 64 / T; gg++) {
+        const double4 o = sacc[gg * T + lane];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tfirst + lane];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tfirst + lane] = tv;
+    }
+  };
+  for (int tbase = 0; tbase < tn; tbase += 64) {
+    const int nt = min(64, tn - tbase);
+    const TargetPasses tp(nt);
+    const int GA = 64 / tp.TA, GB = 64 / tp.TB;
+    const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
+    const double4 xa = ia < nt ? body[tb0 + tbase + ia] : make_double4(0, 0, 0, 0);
+    const double4 xb = ib < tp.r ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
+    double4 acca = make_double4(0, 0, 0, 0), accb = make_double4(0, 0, 0, 0);
     SrcWalker w{e0, 0, e0, 0};
     while (w.e < e1) {
       // Fill: consecutive source leaves (or parts of one) up to kP2PStage
@@ -145,40 +195,14 @@ __global__ void __launch_bounds__(64)
         sb[k] = v;
       });
 #pragma unroll 4
-      for (int k = g; k < cnt; k += G) {
-        const double4 xj = sb[k];
-        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double rs = rsqrt_pos(r2);  // unconditional: a select, not a branch, skips R2 == 0
-        const double invr = r2 > 0 ? rs : 0.0;
-        const double qinvr = xj.w * invr;
-        const double qinvr3 = qinvr * invr * invr;
-        pot += qinvr;
-        ax += dx * qinvr3;
-        ay += dy * qinvr3;
-        az += dz * qinvr3;
+      for (int k = ga; k < cnt; k += GA) pair(xa, sb[k], acca);
+      if (tp.r > 0) {
+#pragma unroll 4
+        for (int k = gb; k < cnt; k += GB) pair(xb, sb[k], accb);
       }
     }
-    // Combine the G group partials in group order.
-    __syncthreads();
-    sacc[lane] = make_double4(pot, ax, ay, az);
-    __syncthreads();
-    if (g == 0 && active) {
-      double4 s = sacc[ti];
-      for (int gg = 1; gg < G; gg++) {
-        const double4 o = sacc[gg * T + ti];
-        s.x += o.x;
-        s.y += o.y;
-        s.z += o.z;
-        s.w += o.w;
-      }
-      double4 tv = trg[tb0 + tbase + ti];
-      tv.x += s.x;
-      tv.y -= s.y;
-      tv.z -= s.z;
-      tv.w -= s.w;
-      trg[tb0 + tbase + ti] = tv;
-    }
+    combine(acca, tp.TA, min(nt, tp.TA), tbase);
+    if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
   }
 }
 
@@ -194,6 +218,7 @@ __global__ void __launch_bounds__(64)
 // kahan<T> (kahan.h:35-41), applied per run instead of per term.  The
 // compensated value is read out as s - c.
 constexpr int kP2PStageF = 256;
+static_assert(kP2PStageF % 64 == 0, "stage tiles are padded to multiples of 64");
 constexpr int kSeg = 8;  // FP32 pair terms summed before each compensated add
 
 struct Kahan32 {
@@ -220,17 +245,80 @@ __global__ void __launch_bounds__(64)
   const int lane = threadIdx.x;
   const int tb0 = cBody[c], tn = cNBody[c];
   const double4 Xc = cX[c];
-  const int T = tn <= 16 ? 16 : (tn <= 32 ? 32 : 64);
-  const int G = 64 / T;
-  const int g = lane / T, ti = lane % T;
   const int e0 = off[c], e1 = off[c + 1];
-  for (int tbase = 0; tbase < tn; tbase += T) {
-    const bool active = tbase + ti < tn;
-    const double4 xd = active ? body[tb0 + tbase + ti] : Xc;
+  // A target's position relative to the leaf centre, FP32 hi + lo.
+  struct Tgt {
+    float x, y, z, xl, yl, zl;
+  };
+  auto load_tgt = [&](bool active, int b) {
+    const double4 xd = active ? body[b] : Xc;
     const double rx = xd.x - Xc.x, ry = xd.y - Xc.y, rz = xd.z - Xc.z;
-    const float xi = (float)rx, yi = (float)ry, zi = (float)rz;
-    const float xil = (float)(rx - xi), yil = (float)(ry - yi), zil = (float)(rz - zi);
-    Kahan32 kp, kx, ky, kz;
+    Tgt t;
+    t.x = (float)rx;
+    t.y = (float)ry;
+    t.z = (float)rz;
+    t.xl = (float)(rx - t.x);
+    t.yl = (float)(ry - t.y);
+    t.zl = (float)(rz - t.z);
+    return t;
+  };
+  struct Acc {
+    Kahan32 p, x, y, z;
+  };
+  // kSeg sources k0, k0 + G, ... in FP32, then one compensated add each.
+  auto segment = [&](const Tgt &t, int k0, int G, Acc &a) {
+    float pot = 0, ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int u = 0; u < kSeg; u++) {
+      const int k = k0 + u * G;
+      const float4 xj = sb[k];
+      const float4 xl = sbl[k];
+      const float dx = (t.x - xj.x) + (t.xl - xl.x), dy = (t.y - xj.y) + (t.yl - xl.y),
+                  dz = (t.z - xj.z) + (t.zl - xl.z);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rs = __builtin_amdgcn_rsqf(r2);
+      const float invr = r2 > 0.0f ? rs : 0.0f;
+      const float qinvr = xj.w * invr;
+      const float qinvr3 = qinvr * invr * invr;
+      pot += qinvr;
+      ax += dx * qinvr3;
+      ay += dy * qinvr3;
+      az += dz * qinvr3;
+    }
+    a.p.add(pot);
+    a.x.add(ax);
+    a.y.add(ay);
+    a.z.add(az);
+  };
+  auto combine = [&](const Acc &a, int T, int n, int tfirst) {
+    __syncthreads();
+    sacc[lane] = make_double4(a.p.value(), a.x.value(), a.y.value(), a.z.value());
+    __syncthreads();
+    if (lane < n) {
+      double4 s = sacc[lane];
+      for (int gg = 1; gg < 64 / T; gg++) {
+        const double4 o = sacc[gg * T + lane];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tfirst + lane];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tfirst + lane] = tv;
+    }
+  };
+  for (int tbase = 0; tbase < tn; tbase += 64) {
+    const int nt = min(64, tn - tbase);
+    const TargetPasses tp(nt);
+    const int GA = 64 / tp.TA, GB = 64 / tp.TB;
+    const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
+    const Tgt ta = load_tgt(ia < nt, tb0 + tbase + ia);
+    const Tgt tb = load_tgt(ib < tp.r, tb0 + tbase + tp.TA + ib);
+    Acc acca, accb;
     SrcWalker w{e0, 0, e0, 0};
     while (w.e < e1) {
       const int cnt = fill_stage<kP2PStageF>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
@@ -242,57 +330,20 @@ __global__ void __launch_bounds__(64)
         sb[k] = make_float4(hx, hy, hz, (float)v.w);
         sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
       });
-      // Pad the tile to whole segments with zero charges (they add exactly 0).
-      const int cntp = (cnt + kSeg * G - 1) / (kSeg * G) * (kSeg * G);
+      // Pad the tile to a multiple of 64 (= kSeg G for G = 8, and a
+      // multiple of it for G < 8) with zero charges, which add exactly 0.
+      const int cntp = (cnt + 63) / 64 * 64;
       for (int k = cnt + lane; k < cntp; k += 64) {
         sb[k] = make_float4(0, 0, 0, 0);
         sbl[k] = make_float4(0, 0, 0, 0);
       }
       __syncthreads();
-      for (int k0 = g; k0 < cntp; k0 += kSeg * G) {
-        float pot = 0, ax = 0, ay = 0, az = 0;
-#pragma unroll
-        for (int u = 0; u < kSeg; u++) {
-          const int k = k0 + u * G;
-          const float4 xj = sb[k];
-          const float4 xl = sbl[k];
-          const float dx = (xi - xj.x) + (xil - xl.x), dy = (yi - xj.y) + (yil - xl.y),
-                      dz = (zi - xj.z) + (zil - xl.z);
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          const float rs = __builtin_amdgcn_rsqf(r2);
-          const float invr = r2 > 0.0f ? rs : 0.0f;
-          const float qinvr = xj.w * invr;
-          const float qinvr3 = qinvr * invr * invr;
-          pot += qinvr;
-          ax += dx * qinvr3;
-          ay += dy * qinvr3;
-          az += dz * qinvr3;
-        }
-        kp.add(pot);
-        kx.add(ax);
-        ky.add(ay);
-        kz.add(az);
-      }
+      for (int k0 = ga; k0 < cntp; k0 += kSeg * GA) segment(ta, k0, GA, acca);
+      if (tp.r > 0)
+        for (int k0 = gb; k0 < cntp; k0 += kSeg * GB) segment(tb, k0, GB, accb);
     }
-    __syncthreads();
-    sacc[lane] = make_double4(kp.value(), kx.value(), ky.value(), kz.value());
-    __syncthreads();
-    if (g == 0 && active) {
-      double4 s = sacc[ti];
-      for (int gg = 1; gg < G; gg++) {
-        const double4 o = sacc[gg * T + ti];
-        s.x += o.x;
-        s.y += o.y;
-        s.z += o.z;
-        s.w += o.w;
-      }
-      double4 tv = trg[tb0 + tbase + ti];
-      tv.x += s.x;
-      tv.y -= s.y;
-      tv.z -= s.z;
-      tv.w -= s.w;
-      trg[tb0 + tbase + ti] = tv;
-    }
+    combine(acca, tp.TA, min(nt, tp.TA), tbase);
+    if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
   }
 }
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "exafmm_hip_traverse", "exafmm_hip_downward_pass", "exafmm_hip_evaluate", "exafmm_hip_get_cells",
     "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
     "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
-    "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald",
+    "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald", "exafmm_hip_count_p2p_bodies",
 )
 
 
@@ -96,6 +96,7 @@ def load():
     L.exafmm_hip_direct_arrays.argtypes = [vp, ip, vp, ip, vp, vp]
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
+    L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]
     L.exafmm_hip_get_dipole.argtypes = [vp, vp, vp]
     L.exafmm_hip_dipole_correction.argtypes = [vp, vp, ip, dp]
     L.exafmm_hip_ewald.argtypes = [vp, ip, dp, dp, dp, dp, ip, vp]
@@ -267,6 +268,12 @@ class FMM:
         s = Stats()
         self._check(self.lib.exafmm_hip_get_stats(self.h, ctypes.byref(s)), "get_stats")
         return s.as_dict()
+
+    def count_p2p_bodies(self):
+        """Target-source body pairs of the current P2P list."""
+        n = ctypes.c_int64(0)
+        self._check(self.lib.exafmm_hip_count_p2p_bodies(self.h, ctypes.byref(n)), "count_p2p_bodies")
+        return int(n.value)
 
 
 def rel_l2(a, b):

@@ -60,7 +60,7 @@ typedef struct {
   double t_bounds, t_build, t_upward, t_lists, t_p2p, t_m2l, t_downward, t_total, t_traverse;
   int64_t num_p2p_pairs;   /* cell pairs in the P2P list                      */
   int64_t num_m2l_pairs;   /* cell pairs in the M2L list                      */
-  int64_t num_p2p_bodies;  /* target-source body pairs evaluated by P2P       */
+  int64_t num_p2p_bodies;  /* body pairs of P2P: set by count_p2p_bodies, -1 */
   int num_cells;
   int num_levels;
   int64_t num_m2l_lattice; /* M2L pairs evaluated as offset-grouped MFMA
@@ -158,6 +158,13 @@ int exafmm_hip_ewald(exafmm_hip_handle *h, int ksize, double alpha, double sigma
                      int parts, double *trg);
 
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
+
+/* Target-source body pairs of the current P2P list (a reduction on the
+ * device; also stored into the stats' num_p2p_bodies until the next
+ * traversal).  Kept out of get_stats so that reading phase times stays cheap.
+ * Extension for measurement (the reference's logger counts them inline,
+ * kernels/LaplaceP2PCPU.cxx). */
+int exafmm_hip_count_p2p_bodies(exafmm_hip_handle *h, int64_t *n);
 
 /* Wait for all work queued on the handle's stream. */
 int exafmm_hip_synchronize(exafmm_hip_handle *h);

@@ -64,6 +64,11 @@ def test_uniform_fmm_matches_oracle(name):
         assert len(gl["m2l_i"]) == z["num_m2l"] and len(gl["p2p_i"]) == z["num_p2p"]
     else:
         assert len(gl["m2l_i"]) == 2 * z["num_m2l"]
+    # Body pairs of the P2P list: sum of nbody(target) * nbody(source).
+    assert f.stats()["num_p2p_bodies"] == -1
+    onb = o.cells()["nbody"].astype(np.int64)
+    assert f.count_p2p_bodies() == int(np.sum(onb[ol["p2p_i"]] * onb[ol["p2p_j"]]))
+    assert f.stats()["num_p2p_bodies"] == f.count_p2p_bodies()
     c, oc = f.cells(), o.cells()
     scale = np.abs(oc["M"]).max()
     assert np.abs(c["M"] - oc["M"]).max() <= 1e-12 * scale
