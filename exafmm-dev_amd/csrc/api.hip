@@ -158,6 +158,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   if (H->h.cfg.nspawn <= 0) H->h.cfg.nspawn = 1000;
   // EXAFMM_HIP_M2L=pairs keeps every M2L pair on the per-pair kernel.
   if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
+  if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
   H->h.nterm = cfg->basis == 1 ? cfg->P * (cfg->P + 1) * (cfg->P + 2) / 6 : cfg->P * (cfg->P + 1) / 2;
   try {
     set_device(H->h);

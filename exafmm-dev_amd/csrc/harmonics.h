@@ -184,4 +184,33 @@ __device__ inline void m2m_lane(int j, int k, const C *sM, const C *sY, double &
   }
 }
 
+// One output coefficient (j, k) of L2L (LaplaceSphericalCPU.cxx:193-224):
+// the parent local expansion sL (packed, m >= 0) shifted by the regular
+// harmonics sY (packed, m >= 0) of X_child - X_parent; adds into (lr, li).
+template <int P, typename C>
+__device__ inline void l2l_lane(int j, int k, const C *sL, const C *sY, double &lr, double &li) {
+  for (int n = j; n < P; n++) {
+    // m from j+k-n to -1: conj(Lp_n^{-m}) Y_{n-j}^{m-k} (-1)^k
+    for (int m = j + k - n; m < 0; m++) {
+      const C s = sL[tri(n) - m];
+      const int a = n - j, b = m - k;  // b < 0
+      C y = sY[tri(a) - b];
+      y.im = -y.im;
+      const double f = (k & 1) ? -1.0 : 1.0;
+      lr += f * (s.re * y.re + s.im * y.im);
+      li += f * (s.re * y.im - s.im * y.re);
+    }
+    for (int m = 0; m <= n; m++) {
+      const int a = n - j, b = m - k;
+      if (a < (b >= 0 ? b : -b)) continue;
+      const C s = sL[tri(n) + m];
+      C y = sY[tri(a) + (b >= 0 ? b : -b)];
+      if (b < 0) y.im = -y.im;
+      const double f = (m < k && ((m - k) & 1)) ? -1.0 : 1.0;
+      lr += f * (s.re * y.re - s.im * y.im);
+      li += f * (s.re * y.im + s.im * y.re);
+    }
+  }
+}
+
 }  // namespace exafmm

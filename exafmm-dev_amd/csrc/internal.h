@@ -121,6 +121,9 @@ struct Handle {
   // Lattice M2L (offset-grouped MFMA contractions) and the CSR of the
   // remaining (irregular) pairs.
   bool lattice_m2l = true;
+  // M2M / L2L as batched MFMA products with per-octant operators.
+  bool updown_mfma = true, ud_ready = false;
+  DBuf<double> ud_ops;
   DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
   DBuf<double> lat_part;
   DBuf<int> lat_pkey, lat_hist, lat_flag, lat_kscan, lat_okey, lat_ookey, lat_tile_tgt, lat_tile_of, lat_col_of, lat_slot, lat_irr,
@@ -175,6 +178,9 @@ void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const do
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
 bool run_m2l_lattice(Handle &h, M2LList &irr);
+bool updown_mfma_enabled(const Handle &h);
+void run_m2m_level_mfma(Handle &h, const int *cells, int cnt);
+void run_l2l_level_mfma(Handle &h, const int *cells, int cnt);
 void run_upward_cart(Handle &h);
 void run_downward_cart(Handle &h);
 void run_m2l_cart(Handle &h);

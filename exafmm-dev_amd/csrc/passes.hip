@@ -17,7 +17,7 @@ __global__ void __launch_bounds__(64)
     k_upward(const int *__restrict__ cells, const double4 *__restrict__ body, double4 *__restrict__ cX,
              const double *__restrict__ cRb, const int *__restrict__ cChild, const int *__restrict__ cNChild,
              const int *__restrict__ cBody, const int *__restrict__ cNBody, cplx *__restrict__ M, double theta,
-             int useRmax) {
+             int useRmax, int m2m) {
   constexpr int NT = Expn<P>::NT;
   constexpr int NO = (NT + 63) / 64;  // coefficients per lane
   __shared__ Geom sg[64];
@@ -65,6 +65,15 @@ __global__ void __launch_bounds__(64)
       }
       __syncthreads();
     }
+  } else if (!m2m) {
+    // M2M runs as a batched MFMA product; only Rmax is needed here.
+    const int ch0 = cChild[c];
+    if (lane == 0)
+      for (int ci = 0; ci < nchild; ci++) {
+        const double4 Xj = cX[ch0 + ci];
+        const double r = exact_dist(Xc.x, Xc.y, Xc.z, Xj.x, Xj.y, Xj.z) + Xj.w;
+        rmax = r > rmax ? r : rmax;
+      }
   } else {
     // M2M (LaplaceSphericalCPU.cxx:120-147): child multipoles shifted to X_c.
     const int ch0 = cChild[c];
@@ -99,7 +108,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int o = 0; o < NO; o++) {
     const int t = lane + 64 * o;
-    if (t < NT) M[(size_t)c * NT + t] = {acc_re[o], acc_im[o]};
+    if (t < NT && (nchild == 0 || m2m)) M[(size_t)c * NT + t] = {acc_re[o], acc_im[o]};
   }
   // setRmax (up_down_pass.h:51-67) and R /= theta (:82).
   sred[lane] = rmax;
@@ -148,28 +157,7 @@ __global__ void __launch_bounds__(64)
       int j, k;
       coef_nm(t, j, k);
       double lr = 0, li = 0;
-      for (int n = j; n < P; n++) {
-        // m from j+k-n to -1: conj(Lp_n^{-m}) Y_{n-j}^{m-k} (-1)^k
-        for (int m = j + k - n; m < 0; m++) {
-          const cplx s = sL[tri(n) - m];
-          const int a = n - j, b = m - k;  // b < 0
-          cplx y = sY[tri(a) - b];
-          y.im = -y.im;
-          const double f = (k & 1) ? -1.0 : 1.0;
-          lr += f * (s.re * y.re + s.im * y.im);
-          li += f * (s.re * y.im - s.im * y.re);
-        }
-        for (int m = 0; m <= n; m++) {
-          const int a = n - j, b = m - k;
-          if (a < (b >= 0 ? b : -b)) continue;
-          const cplx s = sL[tri(n) + m];
-          cplx y = sY[tri(a) + (b >= 0 ? b : -b)];
-          if (b < 0) y.im = -y.im;
-          const double f = (m < k && ((m - k) & 1)) ? -1.0 : 1.0;
-          lr += f * (s.re * y.re - s.im * y.im);
-          li += f * (s.re * y.im + s.im * y.re);
-        }
-      }
+      l2l_lane<P>(j, k, sL, sY, lr, li);
       cplx own = L[(size_t)c * NT + t];
       if (mass) {
         own.re /= m0;
@@ -264,25 +252,35 @@ __global__ void __launch_bounds__(64)
 
 template <int P>
 static void upward_impl(Handle &h) {
+  const bool mf = updown_mfma_enabled(h);
   for (int t = h.nlevels - 1; t >= 0; t--) {
     const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
     if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t], h.body4.p, h.cX.p,
-                       h.cRb.p, h.cChild.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax);
+    const int *cells = h.lvl_cells.p + h.lvl_off[t];
+    hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, h.stream, cells, h.body4.p, h.cX.p, h.cRb.p, h.cChild.p,
+                       h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax, mf ? 0 : 1);
+    if (mf && t + 1 < h.nlevels) run_m2m_level_mfma(h, cells, cnt);
   }
   EXA_HIP(hipGetLastError());
 }
 
 template <int P>
 static void downward_impl(Handle &h) {
+  // MASS mode divides each child's own L by its M_0 before the L2L
+  // (LaplaceSphericalCPU.cxx MASS branch): kept in the per-cell kernel.
+  const bool mf = updown_mfma_enabled(h) && !h.cfg.mass;
   for (int t = 0; t < h.nlevels; t++) {
     const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
     if (cnt == 0) continue;
     // Level 0 is the root: no L2L; L2P only when the root is a leaf
     // (up_down_pass.h:139).
+    if (mf && t > 0) {
+      const int pc = h.lvl_off[t] - h.lvl_off[t - 1];
+      if (pc > 0) run_l2l_level_mfma(h, h.lvl_cells.p + h.lvl_off[t - 1], pc);
+    }
     hipLaunchKernelGGL(k_downward<P>, dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t], h.body4.p,
-                       h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p, t > 0 ? 1 : 0,
-                       h.cfg.mass);
+                       h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p,
+                       t > 0 && !mf ? 1 : 0, h.cfg.mass);
   }
   EXA_HIP(hipGetLastError());
 }

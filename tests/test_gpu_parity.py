@@ -197,3 +197,25 @@ def test_lattice_m2l_matches_per_pair(name, builder, monkeypatch):
     assert sb["num_m2l_lattice"] > 0.5 * sb["num_m2l_pairs"], (sb["num_m2l_lattice"], sb["num_m2l_pairs"])
     assert np.abs(Lb - La).max() <= 1e-12 * np.abs(La).max()
     assert rel_l2(rb, ra) < 1e-13
+
+
+@pytest.mark.parametrize("name,builder", [("cube16k_u_p10", "uniform"), ("plummer16k_a_p10", "adaptive"),
+                                          ("cube4k_a_p12", "adaptive"), ("cube4k_u_p4", "uniform"),
+                                          ("mass_plummer8k_u_p8", "uniform")])
+def test_octant_operator_updown_matches_per_cell(name, builder, monkeypatch):
+    """M2M and L2L as per-octant operator products on the matrix cores give
+    the per-cell kernels' multipole and local expansions (to rounding), on
+    uniform and adaptive trees (children missing from some octants)."""
+    z = load_case(name)
+    out = {}
+    for mode in ("cells", "mfma"):
+        monkeypatch.setenv("EXAFMM_HIP_UPDOWN", mode)
+        f = gpu_fmm(z, builder)
+        f.evaluate()
+        c = f.cells()
+        out[mode] = (c["M"], c["L"], f.results())
+        f.close()
+    (Ma, La, ra), (Mb, Lb, rb) = out["cells"], out["mfma"]
+    assert np.abs(Mb - Ma).max() <= 1e-13 * np.abs(Ma).max()
+    assert np.abs(Lb - La).max() <= 1e-12 * np.abs(La).max()
+    assert rel_l2(rb, ra) < 1e-13
