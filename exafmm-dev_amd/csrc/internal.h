@@ -194,6 +194,7 @@ int64_t count_p2p_body_pairs(Handle &h);
 
 // Scan / sort helpers (hipCUB) in util.hip.
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n);
+void exclusive_scan_int4(Handle &h, const int4 *in, int4 *out, int n);  // component-wise
 void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt);
 void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,

@@ -52,21 +52,21 @@ __device__ inline int classify(const CellView &cv, int ci, int cj, int img, doub
 // stands for both directions: M2L and P2P emit both directed entries, and a
 // cell split against itself visits only the child pairs (a, b), a <= b, as
 // TraverseRange skips the (latter, former) quadrant (traversal.h:108-120).
+// Counts per frontier pair (M2L entries, P2P entries, next-frontier pairs)
+// packed in one int4 so that a single exclusive scan gives all three
+// output offsets.
 __global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
-                           int *__restrict__ act, int *__restrict__ c_m2l, int *__restrict__ c_p2p,
-                           int *__restrict__ c_next) {
+                           int *__restrict__ act, int4 *__restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nf) return;
-  if (i == nf) {  // trailing zero so that the exclusive scans end in the totals
-    c_m2l[i] = c_p2p[i] = c_next[i] = 0;
+  if (i == nf) {  // trailing zero so that the exclusive scan ends in the totals
+    cnt[i] = make_int4(0, 0, 0, 0);
     return;
   }
   const int4 f = fr[i];
   const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
   const int both = (mutual && f.x != f.y) ? 2 : 1;
   act[i] = a;
-  c_m2l[i] = a == 0 ? both : 0;
-  c_p2p[i] = a == 1 ? both : 0;
   int nx = 0;
   if (a == 2) {
     nx = cv.nchild[f.x];
@@ -76,12 +76,11 @@ __global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, dou
     const int nci = cv.nchild[f.x];
     nx = (mutual && f.x == f.y) ? nci * (nci + 1) / 2 : nci * cv.nchild[f.y];
   }
-  c_next[i] = nx;
+  cnt[i] = make_int4(a == 0 ? both : 0, a == 1 ? both : 0, nx, 0);
 }
 
 __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
-                       const int *__restrict__ o_m2l, const int *__restrict__ o_p2p, const int *__restrict__ o_next,
-                       int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                       const int4 *__restrict__ ofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
                        unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
                        unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,7 +89,7 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mut
   const int a = act[i];
   const bool both = mutual && f.x != f.y;
   if (a == 0) {
-    const int64_t o = base_m2l + o_m2l[i];
+    const int64_t o = base_m2l + ofs[i].x;
     m2l_i[o] = f.x;
     m2l_j[o] = f.y;
     m2l_img[o] = (unsigned char)f.z;
@@ -100,7 +99,7 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mut
       m2l_img[o + 1] = (unsigned char)f.z;
     }
   } else if (a == 1) {
-    const int64_t o = base_p2p + o_p2p[i];
+    const int64_t o = base_p2p + ofs[i].y;
     p2p_i[o] = f.x;
     p2p_j[o] = f.y;
     p2p_img[o] = (unsigned char)f.z;
@@ -110,7 +109,7 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mut
       p2p_img[o + 1] = (unsigned char)f.z;
     }
   } else {
-    int o = o_next[i];
+    int o = ofs[i].z;
     if (a == 2) {
       const int c0 = cv.child[f.x], nc = cv.nchild[f.x];
       for (int c = 0; c < nc; c++) next[o++] = make_int4(c0 + c, f.y, f.z, 0);
@@ -153,15 +152,6 @@ __global__ void k_csr_offsets(const int *__restrict__ keys, int64_t n, int nc, i
       hi = mid;
   }
   off[c] = (int)lo;
-}
-
-__global__ void k_totals(const int *__restrict__ a, const int *__restrict__ b, const int *__restrict__ c, int n,
-                         int *__restrict__ out) {
-  if (threadIdx.x == 0) {
-    out[0] = a[n];
-    out[1] = b[n];
-    out[2] = c[n];
-  }
 }
 
 __global__ void k_permute_pairs(const int *__restrict__ order, int64_t n, const int *__restrict__ src_in,
@@ -268,20 +258,17 @@ void run_lists(Handle &h) {
   h.cnt.reserve(4);
   while (nf > 0) {
     h.ia.reserve(nf + 1);  // action
-    h.ib.reserve(3 * ((size_t)nf + 1));  // counts
-    h.ic.reserve(3 * ((size_t)nf + 1));  // scans
+    h.ib.reserve(4 * ((size_t)nf + 1));  // counts (int4)
+    h.ic.reserve(4 * ((size_t)nf + 1));  // offsets (int4)
     int *act = h.ia.p;
-    int *cm = h.ib.p, *cp = h.ib.p + (nf + 1), *cn = h.ib.p + 2 * (nf + 1);
-    int *om = h.ic.p, *op = h.ic.p + (nf + 1), *on = h.ic.p + 2 * (nf + 1);
+    int4 *cnt = reinterpret_cast<int4 *>(h.ib.p), *ofs = reinterpret_cast<int4 *>(h.ic.p);
     hipLaunchKernelGGL(k_classify, dim3(div_up(nf + 1, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
-                       h.cfg.nspawn, mutual, act, cm, cp, cn);
-    exclusive_scan_int(h, cm, om, nf + 1);
-    exclusive_scan_int(h, cp, op, nf + 1);
-    exclusive_scan_int(h, cn, on, nf + 1);
-    int tot[3];
-    hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, h.stream, om, op, on, nf, h.cnt.p);
-    EXA_HIP(hipMemcpyAsync(tot, h.cnt.p, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
+                       h.cfg.nspawn, mutual, act, cnt);
+    exclusive_scan_int4(h, cnt, ofs, nf + 1);
+    int4 tot4;
+    EXA_HIP(hipMemcpyAsync(&tot4, ofs + nf, sizeof(tot4), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
+    const int tot[3] = {tot4.x, tot4.y, tot4.z};
     grow_keep(h, mi, nm + tot[0] + 1, nm);
     grow_keep(h, mj, nm + tot[0] + 1, nm);
     grow_keep(h, mimg, nm + tot[0] + 1, nm);
@@ -289,9 +276,8 @@ void run_lists(Handle &h) {
     grow_keep(h, pj, np + tot[1] + 1, np);
     grow_keep(h, pimg, np + tot[1] + 1, np);
     h.fb.reserve((size_t)tot[2] + 1);
-    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, mutual, act, om, op, on, nm,
-                       np, mi.p,
-                       mj.p, mimg.p, pi.p, pj.p, pimg.p, h.fb.p);
+    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, mutual, act, ofs, nm, np,
+                       mi.p, mj.p, mimg.p, pi.p, pj.p, pimg.p, h.fb.p);
     nm += tot[0];
     np += tot[1];
     nf = tot[2];

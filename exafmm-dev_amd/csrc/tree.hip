@@ -130,6 +130,11 @@ __global__ void k_gather_bodies(const double4 *__restrict__ in, const int *__res
 // Head flags of tree level t: body i starts a level-t cell when its key
 // prefix (key >> shift) differs from body i-1's.  flags has n+1 entries; the
 // last is 0 so that the exclusive scan yields the total at index n.
+// out[i] = in[i * stride], i < n (n small).
+__global__ void k_strided_gather(const int *__restrict__ in, int64_t stride, int n, int *__restrict__ out) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = in[(int64_t)i * stride];
+}
+
 __global__ void k_level_heads(const uint64_t *__restrict__ key, int n, int shift, int *__restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
@@ -252,10 +257,10 @@ void run_build_uniform(Handle &h) {
     hipLaunchKernelGGL(k_level_heads, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, key, n, 3 * (L - t), fl);
     exclusive_scan_int(h, fl, sc, n + 1);
   }
-  // Totals: scan[n] of each level.
+  // Totals: scan[n] of each level, gathered on the device, one read-back.
   h.cnt.reserve(L + 1);
-  for (int t = 0; t <= L; t++)
-    EXA_HIP(hipMemcpyAsync(&tot[t], h.id.p + stride * t + n, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  hipLaunchKernelGGL(k_strided_gather, dim3(1), dim3(64), 0, h.stream, h.id.p + n, (int64_t)stride, L + 1, h.cnt.p);
+  EXA_HIP(hipMemcpyAsync(tot.data(), h.cnt.p, (L + 1) * sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
   int nc = 0;
   std::vector<int> off(L + 2, 0);

@@ -12,10 +12,10 @@
 // evaluated at R = 1.  They are built once per handle on the device by
 // applying the same per-coefficient code (m2m_lane, l2l_lane) to unit
 // vectors, and stored in MFMA fragment order.  Each workgroup takes 16
-// parent cells of one level (the MFMA columns) and 4 waves, wave w handling
-// octants 2w and 2w + 1: M2M sums the 4 waves' partials in wave order
-// (deterministic), L2L writes each child's rows from the wave that owns its
-// octant.  Zero blocks (M2M: child degree > parent degree; L2L: parent
+// parent cells of one level (the MFMA columns).  M2M: 4 waves, wave w
+// handling octants 2w and 2w + 1, partials summed in a fixed order
+// (deterministic).  L2L: 8 waves, wave o writing the rows of the children in
+// octant o.  Zero blocks (M2M: child degree > parent degree; L2L: parent
 // degree < child degree) are skipped at compile time.
 #include "harmonics.h"
 #include "internal.h"
@@ -62,6 +62,34 @@ __device__ inline void octant_unit(int o, double &ux, double &uy, double &uz) {
   ux = (o & 1) ? 1.0 : -1.0;
   uy = (o & 2) ? 1.0 : -1.0;
   uz = (o & 4) ? 1.0 : -1.0;
+}
+
+// acc += Op * b for one octant's operator in fragment order (row blocks
+// rb, column chunks [lo(rb), hi(rb)) that are not identically zero).  The
+// next row block's operator entries are loaded while the current block's
+// MFMAs run, so each wave waits on memory once per operator, not per MFMA.
+template <int P, bool M2M>
+__device__ inline void op_apply(const double *__restrict__ Op, int lane, const double (&b)[UDShape<P>::KC],
+                                v4d_ud (&acc)[UDShape<P>::RB]) {
+  using U = UDShape<P>;
+  constexpr int KC = U::KC, RB = U::RB;
+  auto lo = [](int rb) { return M2M ? 0 : U::l2l_kbeg(rb); };
+  auto hi = [](int rb) { return M2M ? U::m2m_kend(rb) : KC; };
+  double cur[KC], nxt[KC];
+#pragma unroll
+  for (int kc = lo(0); kc < hi(0); kc++) cur[kc] = Op[kc * 64 + lane];
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++) {
+    if (rb + 1 < RB) {
+#pragma unroll
+      for (int kc = lo(rb + 1); kc < hi(rb + 1); kc++) nxt[kc] = Op[((rb + 1) * KC + kc) * 64 + lane];
+    }
+#pragma unroll
+    for (int kc = lo(rb); kc < hi(rb); kc++)
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[kc], b[kc], acc[rb], 0, 0, 0);
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) cur[kc] = nxt[kc];
+  }
 }
 
 // Column q (real input index) of A_o (op 0) or B_o (op 1): grid (NR, 8, 2),
@@ -181,12 +209,7 @@ __global__ void __launch_bounds__(256)
       const double v = src[k < NR ? k : NR - 1] * s.ipw[col][deg.d[k]];
       b[kc] = (c >= 0 && k < NR) ? v : 0.0;
     }
-    const double *Ao = A + (size_t)o * RB * KC * 64;
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-      for (int kc = 0; kc < U::m2m_kend(rb); kc++)
-        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ao[(rb * KC + kc) * 64 + lane], b[kc], acc[rb], 0, 0, 0);
+    op_apply<P, true>(A + (size_t)o * RB * KC * 64, lane, b, acc);
   }
   // Octant partials summed as (w0 + w2) + (w1 + w3).
   auto stash = [&](int slot) {
@@ -228,7 +251,7 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int P>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
     k_l2l_mfma(const int *__restrict__ cells, int ncell, const double4 *__restrict__ cX, const double *__restrict__ cRb,
                const int *__restrict__ cChild, const int *__restrict__ cNChild, const double *__restrict__ B,
                cplx *__restrict__ L) {
@@ -237,7 +260,7 @@ __global__ void __launch_bounds__(256)
   constexpr DegTab<P> deg{};
   __shared__ UDCols<P> s;
   ud_setup<P>(s, cells, ncell, cX, cRb, cChild, cNChild);
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, o = tid >> 6, lane = tid & 63;  // wave o: octant o
   const int col = lane & 15, kq = lane >> 4;
   double *Lr = reinterpret_cast<double *>(L);
   const int p = s.cell[col];
@@ -249,28 +272,19 @@ __global__ void __launch_bounds__(256)
     const double v = src[k < NR ? k : NR - 1] * s.pw[col][deg.d[k]];
     b[kc] = (p >= 0 && k < NR) ? v : 0.0;
   }
-#pragma unroll 1
-  for (int oo = 0; oo < 2; oo++) {
-    const int o = 2 * wave + oo;
-    const double *Bo = B + (size_t)o * RB * KC * 64;
-    v4d_ud acc[RB];
+  v4d_ud acc[RB];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
-      acc[rb] = v4d_ud{0, 0, 0, 0};
+  for (int rb = 0; rb < RB; rb++) acc[rb] = v4d_ud{0, 0, 0, 0};
+  op_apply<P, false>(B + (size_t)o * RB * KC * 64, lane, b, acc);
+  const int c = s.ch[col][o];
+  if (c < 0 || p < 0) return;
 #pragma unroll
-      for (int kc = U::l2l_kbeg(rb); kc < KC; kc++)
-        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bo[(rb * KC + kc) * 64 + lane], b[kc], acc[rb], 0, 0, 0);
+  for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = rb * 16 + (lane >> 4) + 4 * r;
+      if (row < NR) Lr[(size_t)c * NR + row] += s.zero[col] ? src[row] : acc[rb][r] * s.ipw[col][deg.d[row]];
     }
-    const int c = s.ch[col][o];
-    if (c < 0 || p < 0) continue;
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = rb * 16 + (lane >> 4) + 4 * r;
-        if (row < NR) Lr[(size_t)c * NR + row] += s.zero[col] ? src[row] : acc[rb][r] * s.ipw[col][deg.d[row]];
-      }
-  }
 }
 
 __global__ void k_fill_f64(double *__restrict__ a, int64_t n) {
@@ -301,7 +315,7 @@ template <int P>
 static void l2l_level(Handle &h, const int *cells, int cnt) {
   using U = UDShape<P>;
   ensure_ops<P>(h);
-  hipLaunchKernelGGL(k_l2l_mfma<P>, dim3(div_up(cnt, 16)), dim3(256), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
+  hipLaunchKernelGGL(k_l2l_mfma<P>, dim3(div_up(cnt, 16)), dim3(512), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
                      h.cChild.p, h.cNChild.p, h.ud_ops.p + U::OPS, h.L.p);
 }
 

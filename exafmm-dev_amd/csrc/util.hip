@@ -70,6 +70,20 @@ void exclusive_scan_int(Handle &h, const int *in, int *out, int n) {
   EXA_HIP(hipcub::DeviceScan::ExclusiveSum(h.tmp.p, bytes, in, out, n, h.stream));
 }
 
+struct Int4Sum {
+  __host__ __device__ int4 operator()(const int4 &a, const int4 &b) const {
+    return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+  }
+};
+
+void exclusive_scan_int4(Handle &h, const int4 *in, int4 *out, int n) {
+  size_t bytes = 0;
+  const int4 zero = make_int4(0, 0, 0, 0);
+  EXA_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, Int4Sum{}, zero, n, h.stream));
+  h.tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceScan::ExclusiveScan(h.tmp.p, bytes, in, out, Int4Sum{}, zero, n, h.stream));
+}
+
 void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt) {
   hipcub::DoubleBuffer<uint64_t> k(keys, keys_alt);
