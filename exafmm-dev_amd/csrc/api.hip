@@ -1,6 +1,7 @@
 // C-ABI of libexafmm_hip (declared in include/exafmm_hip.h).  Every entry
 // point converts exceptions into status codes and keeps the message on the
 // handle.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -99,6 +100,10 @@ static void do_traverse(Handle &h) {
   run_p2p(h, h.stream2);
   EXA_HIP(hipEventRecord(h.ev[10], h.stream2));
   EXA_HIP(hipEventRecord(h.ev[6], h.stream));
+  if (h.mult_pending) {  // multipoles from the split upward pass
+    EXA_HIP(hipStreamWaitEvent(h.stream, h.ev_mult, 0));
+    h.mult_pending = false;
+  }
   run_m2l(h);
   if (h.cfg.images > 0) run_periodic(h);
   EXA_HIP(hipEventRecord(h.ev[11], h.stream));
@@ -165,6 +170,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream2, hipStreamNonBlocking));
     for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));
+    EXA_HIP(hipEventCreate(&H->h.ev_mult));
   } catch (const Error &e) {
     delete H;
     return e.code;
@@ -259,7 +265,17 @@ int exafmm_hip_evaluate(exafmm_hip_handle *H) {
     EXA_HIP(hipEventRecord(h.ev[1], h.stream));
     do_build(h);
     EXA_HIP(hipEventRecord(h.ev[2], h.stream));
-    do_upward(h);
+    // Inside one evaluation the multipoles (P2M, M2M) run on the second
+    // stream while the main stream computes radii and builds the lists.
+    const bool split = h.cfg.basis == 0 && !h.cfg.useRopt;
+    if (split) {
+      ensure(h.have_tree, "upward_pass: tree not built");
+      run_upward_split(h);
+      h.have_up = true;
+      h.mult_pending = true;
+    } else {
+      do_upward(h);
+    }
     EXA_HIP(hipEventRecord(h.ev[3], h.stream));
     do_traverse(h);
     run_downward(h);
@@ -268,7 +284,8 @@ int exafmm_hip_evaluate(exafmm_hip_handle *H) {
     exafmm_hip_stats &s = h.stats;
     s.t_bounds = elapsed(h.ev[0], h.ev[1]);
     s.t_build = elapsed(h.ev[1], h.ev[2]);
-    s.t_upward = elapsed(h.ev[2], h.ev[3]);
+    s.t_upward = elapsed(h.ev[2], h.ev[3]);  // split: the longer of radii and multipoles
+    if (split) s.t_upward = std::max(s.t_upward, (double)elapsed(h.ev[17], h.ev_mult));
     s.t_lists = elapsed(h.ev[4], h.ev[5]);
     s.t_p2p = elapsed(h.ev[9], h.ev[10]);   // on the P2P stream
     s.t_m2l = elapsed(h.ev[6], h.ev[11]);   // on the main stream

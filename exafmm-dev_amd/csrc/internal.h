@@ -73,7 +73,9 @@ struct Handle {
   std::string err;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // P2P runs here, concurrently with M2L
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[18] = {};
+  hipEvent_t ev_mult = nullptr;  // multipoles done on stream2 (split upward pass)
+  bool mult_pending = false;     // the main stream has not joined ev_mult yet
 
   // Bodies.
   int n = 0;
@@ -169,6 +171,7 @@ void run_bounds(Handle &h);
 void run_build_uniform(Handle &h);
 void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
+void run_upward_split(Handle &h);
 void run_lists(Handle &h);
 void run_p2p(Handle &h, hipStream_t st);
 void run_m2l(Handle &h);
@@ -179,7 +182,7 @@ void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
 bool run_m2l_lattice(Handle &h, M2LList &irr);
 bool updown_mfma_enabled(const Handle &h);
-void run_m2m_level_mfma(Handle &h, const int *cells, int cnt);
+void run_m2m_level_mfma(Handle &h, const int *cells, int cnt, hipStream_t st);
 void run_l2l_level_mfma(Handle &h, const int *cells, int cnt);
 void run_upward_cart(Handle &h);
 void run_downward_cart(Handle &h);

@@ -12,16 +12,49 @@
 
 namespace exafmm {
 
+constexpr int kP2MChunk = 32;  // bodies per P2M table chunk
+
+// Row t (real index) of one body's P2M table, column b: q conj(R_n^m(g)) for
+// every n, m >= 0, from the recurrence of regular_nm (harmonics.h) run once
+// per order m: h_m^m = (r sin(alpha)/2)^m / m!,
+// h_n^m = -((2n-1) r cos(alpha) h_{n-1}^m + r^2 h_{n-2}^m) / (n^2 - m^2),
+// R_n^m = h_n^m e^{i m beta}.
+template <int P>
+__device__ inline void p2m_rows(const Geom &g, double q, int b, double (*sR)[kP2MChunk + 1]) {
+  const double rs = 0.5 * g.r * g.st, cr = g.ct * g.r, r2 = g.r * g.r;
+  double pm = q, cm = 1, sm = 0;  // q (rs)^m / m!, e^{i m beta}
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    double hp = 0, h = pm;
+#pragma unroll
+    for (int n = m; n < P; n++) {
+      if (n > m) {
+        const double hn = -((2 * n - 1) * cr * h + r2 * hp) * (1.0 / (n * n - m * m));
+        hp = h;
+        h = hn;
+      }
+      const int t = n * (n + 1) / 2 + m;
+      sR[2 * t][b] = h * cm;
+      sR[2 * t + 1][b] = -h * sm;
+    }
+    pm *= rs * (1.0 / (m + 1));
+    const double c1 = cm * g.cp - sm * g.sp;
+    sm = sm * g.cp + cm * g.sp;
+    cm = c1;
+  }
+}
+
 template <int P>
 __global__ void __launch_bounds__(64)
     k_upward(const int *__restrict__ cells, const double4 *__restrict__ body, double4 *__restrict__ cX,
              const double *__restrict__ cRb, const int *__restrict__ cChild, const int *__restrict__ cNChild,
              const int *__restrict__ cBody, const int *__restrict__ cNBody, cplx *__restrict__ M, double theta,
-             int useRmax, int m2m) {
+             int useRmax, int m2m, int what) {
+  // what: 1 = multipoles (P2M; M2M unless it runs as an MFMA product),
+  //       2 = radii (setRmax, R /= theta), 3 = both.
   constexpr int NT = Expn<P>::NT;
   constexpr int NO = (NT + 63) / 64;  // coefficients per lane
-  __shared__ Geom sg[64];
-  __shared__ double sq[64];
+  __shared__ double sR[2 * NT][kP2MChunk + 1];  // P2M rows (padded against bank conflicts)
   __shared__ cplx sY[NT];
   __shared__ cplx sM[NT];
   __shared__ double sred[64];
@@ -38,35 +71,43 @@ __global__ void __launch_bounds__(64)
   }
   double rmax = 0;
   if (nchild == 0) {
-    // P2M: M_n^m += q conj(R_n^m(x_b - X_c)) over the cell's bodies, in order.
     const int b0 = cBody[c], nb = cNBody[c];
-    for (int base = 0; base < nb; base += 64) {
-      const int cnt = min(64, nb - base);
-      if (lane < cnt) {
-        const double4 v = body[b0 + base + lane];
-        sg[lane] = make_geom(v.x - Xc.x, v.y - Xc.y, v.z - Xc.z);
-        sq[lane] = v.w;
+    if (!(what & 1)) {
+      for (int i = lane; i < nb; i += 64) {
+        const double4 v = body[b0 + i];
         const double r = exact_dist(Xc.x, Xc.y, Xc.z, v.x, v.y, v.z);
         rmax = r > rmax ? r : rmax;
       }
-      __syncthreads();
-      for (int k = 0; k < cnt; k++) {
-        const Geom g = sg[k];
-        const double q = sq[k];
+    } else {
+      // P2M: M_n^m += q conj(R_n^m(x_b - X_c)) over the cell's bodies, in
+      // order.  Chunks of kP2MChunk bodies: each body's lane writes its
+      // charge-weighted table (all n, m by recurrence) into LDS, then lanes
+      // own coefficients and add the chunk's rows in body order.
+      for (int base = 0; base < nb; base += kP2MChunk) {
+        const int cnt = min(kP2MChunk, nb - base);
+        __syncthreads();  // the previous chunk's rows are consumed
+        if (lane < cnt) {
+          const double4 v = body[b0 + base + lane];
+          if (what & 2) {
+            const double r = exact_dist(Xc.x, Xc.y, Xc.z, v.x, v.y, v.z);
+            rmax = r > rmax ? r : rmax;
+          }
+          p2m_rows<P>(make_geom(v.x - Xc.x, v.y - Xc.y, v.z - Xc.z), v.w, lane, sR);
+        }
+        __syncthreads();
 #pragma unroll
         for (int o = 0; o < NO; o++) {
-          if (lane + 64 * o < NT) {
-            double re, im;
-            regular_nm(tn[o], tm[o], g, re, im);
-            acc_re[o] += q * re;
-            acc_im[o] -= q * im;
-          }
+          const int t = lane + 64 * o;
+          if (t < NT)
+            for (int k = 0; k < cnt; k++) {
+              acc_re[o] += sR[2 * t][k];
+              acc_im[o] += sR[2 * t + 1][k];
+            }
         }
       }
-      __syncthreads();
     }
-  } else if (!m2m) {
-    // M2M runs as a batched MFMA product; only Rmax is needed here.
+  } else if (!m2m || !(what & 1)) {
+    // M2M runs as a batched MFMA product (or is not asked for): Rmax only.
     const int ch0 = cChild[c];
     if (lane == 0)
       for (int ci = 0; ci < nchild; ci++) {
@@ -108,8 +149,9 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int o = 0; o < NO; o++) {
     const int t = lane + 64 * o;
-    if (t < NT && (nchild == 0 || m2m)) M[(size_t)c * NT + t] = {acc_re[o], acc_im[o]};
+    if (t < NT && (what & 1) && (nchild == 0 || m2m)) M[(size_t)c * NT + t] = {acc_re[o], acc_im[o]};
   }
+  if (!(what & 2)) return;
   // setRmax (up_down_pass.h:51-67) and R /= theta (:82).
   sred[lane] = rmax;
   __syncthreads();
@@ -251,15 +293,15 @@ __global__ void __launch_bounds__(64)
 }
 
 template <int P>
-static void upward_impl(Handle &h) {
+static void upward_impl(Handle &h, int what, hipStream_t st) {
   const bool mf = updown_mfma_enabled(h);
   for (int t = h.nlevels - 1; t >= 0; t--) {
     const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
     if (cnt == 0) continue;
     const int *cells = h.lvl_cells.p + h.lvl_off[t];
-    hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, h.stream, cells, h.body4.p, h.cX.p, h.cRb.p, h.cChild.p,
-                       h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax, mf ? 0 : 1);
-    if (mf && t + 1 < h.nlevels) run_m2m_level_mfma(h, cells, cnt);
+    hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, st, cells, h.body4.p, h.cX.p, h.cRb.p, h.cChild.p,
+                       h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax, mf ? 0 : 1, what);
+    if ((what & 1) && mf && t + 1 < h.nlevels) run_m2m_level_mfma(h, cells, cnt, st);
   }
   EXA_HIP(hipGetLastError());
 }
@@ -295,12 +337,37 @@ static void downward_impl(Handle &h) {
     default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");    \
   }
 
+#define EXA_DISPATCH_P3(fn, h, a, b)                                                 \
+  switch ((h).cfg.P) {                                                               \
+    case 4: fn<4>(h, a, b); break;                                                   \
+    case 6: fn<6>(h, a, b); break;                                                   \
+    case 8: fn<8>(h, a, b); break;                                                   \
+    case 10: fn<10>(h, a, b); break;                                                 \
+    case 12: fn<12>(h, a, b); break;                                                 \
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");    \
+  }
+
 void run_upward(Handle &h) {
   h.M.resize((size_t)h.ncells * h.nterm);
   h.L.resize((size_t)h.ncells * h.nterm);
   EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
   if (h.cfg.basis == 1) return run_upward_cart(h);
-  EXA_DISPATCH_P(upward_impl, h);
+  EXA_DISPATCH_P3(upward_impl, h, 3, h.stream);
+}
+
+// The upward pass split over the two streams: radii (needed by the
+// traversal) on the main stream, multipoles (needed only by M2L) on the
+// second stream, which the caller joins through h.ev_mult before M is read.
+void run_upward_split(Handle &h) {
+  h.M.resize((size_t)h.ncells * h.nterm);
+  h.L.resize((size_t)h.ncells * h.nterm);
+  EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
+  EXA_HIP(hipEventRecord(h.ev[16], h.stream));
+  EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[16], 0));
+  EXA_HIP(hipEventRecord(h.ev[17], h.stream2));
+  EXA_DISPATCH_P3(upward_impl, h, 1, h.stream2);
+  EXA_HIP(hipEventRecord(h.ev_mult, h.stream2));
+  EXA_DISPATCH_P3(upward_impl, h, 2, h.stream);
 }
 
 void run_downward(Handle &h) {

@@ -292,29 +292,32 @@ __global__ void k_fill_f64(double *__restrict__ a, int64_t n) {
   if (i < n) a[i] = 0;
 }
 
+// Built once per handle, on the stream of the first product that needs
+// them (later users on the other stream are ordered after it by the
+// evaluation's stream joins).
 template <int P>
-static void ensure_ops(Handle &h) {
+static void ensure_ops(Handle &h, hipStream_t st) {
   using U = UDShape<P>;
   if (h.ud_ready) return;
   h.ud_ops.resize(2 * (size_t)U::OPS);
-  hipLaunchKernelGGL(k_fill_f64, dim3(div_up(2 * (int64_t)U::OPS, 256)), dim3(256), 0, h.stream, h.ud_ops.p,
+  hipLaunchKernelGGL(k_fill_f64, dim3(div_up(2 * (int64_t)U::OPS, 256)), dim3(256), 0, st, h.ud_ops.p,
                      2 * (int64_t)U::OPS);
-  hipLaunchKernelGGL(k_ud_ops<P>, dim3(U::NR, 8, 2), dim3(64), 0, h.stream, h.ud_ops.p, h.ud_ops.p + U::OPS);
+  hipLaunchKernelGGL(k_ud_ops<P>, dim3(U::NR, 8, 2), dim3(64), 0, st, h.ud_ops.p, h.ud_ops.p + U::OPS);
   EXA_HIP(hipGetLastError());
   h.ud_ready = true;
 }
 
 template <int P>
-static void m2m_level(Handle &h, const int *cells, int cnt) {
-  ensure_ops<P>(h);
-  hipLaunchKernelGGL(k_m2m_mfma<P>, dim3(div_up(cnt, 16)), dim3(256), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
+static void m2m_level(Handle &h, const int *cells, int cnt, hipStream_t st) {
+  ensure_ops<P>(h, st);
+  hipLaunchKernelGGL(k_m2m_mfma<P>, dim3(div_up(cnt, 16)), dim3(256), 0, st, cells, cnt, h.cX.p, h.cRb.p,
                      h.cChild.p, h.cNChild.p, h.ud_ops.p, h.M.p);
 }
 
 template <int P>
 static void l2l_level(Handle &h, const int *cells, int cnt) {
   using U = UDShape<P>;
-  ensure_ops<P>(h);
+  ensure_ops<P>(h, h.stream);
   hipLaunchKernelGGL(k_l2l_mfma<P>, dim3(div_up(cnt, 16)), dim3(512), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
                      h.cChild.p, h.cNChild.p, h.ud_ops.p + U::OPS, h.L.p);
 }
@@ -325,13 +328,13 @@ static void l2l_level(Handle &h, const int *cells, int cnt) {
 // against the reference there.
 bool updown_mfma_enabled(const Handle &h) { return h.cfg.basis == 0 && h.cfg.images == 0 && h.updown_mfma; }
 
-void run_m2m_level_mfma(Handle &h, const int *cells, int cnt) {
+void run_m2m_level_mfma(Handle &h, const int *cells, int cnt, hipStream_t st) {
   switch (h.cfg.P) {
-    case 4: m2m_level<4>(h, cells, cnt); break;
-    case 6: m2m_level<6>(h, cells, cnt); break;
-    case 8: m2m_level<8>(h, cells, cnt); break;
-    case 10: m2m_level<10>(h, cells, cnt); break;
-    case 12: m2m_level<12>(h, cells, cnt); break;
+    case 4: m2m_level<4>(h, cells, cnt, st); break;
+    case 6: m2m_level<6>(h, cells, cnt, st); break;
+    case 8: m2m_level<8>(h, cells, cnt, st); break;
+    case 10: m2m_level<10>(h, cells, cnt, st); break;
+    case 12: m2m_level<12>(h, cells, cnt, st); break;
     default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
   }
   EXA_HIP(hipGetLastError());

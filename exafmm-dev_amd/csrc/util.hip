@@ -59,6 +59,7 @@ Handle::~Handle() {
   red.release();
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (ev_mult) (void)hipEventDestroy(ev_mult);
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
 }
