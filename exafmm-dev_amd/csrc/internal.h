@@ -131,6 +131,7 @@ struct Handle {
   DBuf<int> lat_pkey, lat_hist, lat_flag, lat_kscan, lat_okey, lat_ookey, lat_tile_tgt, lat_tile_of, lat_col_of, lat_slot, lat_irr,
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
+  DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
   DBuf<double> lat_tile_w, lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;
   int64_t lat_stats_pairs = 0;

@@ -105,7 +105,8 @@ struct MfmaAcc<float> {
 // Pair classification: key of the lattice offset, or -1.
 __global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *__restrict__ src,
                                const unsigned char *__restrict__ img, int64_t np, const double4 *__restrict__ cX,
-                               const double *__restrict__ cRb, int *__restrict__ pkey, int *__restrict__ hist) {
+                               const double *__restrict__ cRb, int *__restrict__ pkey, int *__restrict__ ptgt,
+                               int *__restrict__ hist) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= np) return;
   int lo = 0, hi = nc;  // largest c with off[c] <= e
@@ -117,6 +118,7 @@ __global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *_
       hi = mid;
   }
   const int t = lo, s = src[e];
+  ptgt[e] = t;
   int key = -1;
   // Same level or one level apart: then the offset's residue mod 4 fixes
   // the level relation (0: same, 2: source one up, odd: source one down).
@@ -244,20 +246,8 @@ __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *_
   if (col == 0) tile_u[t] = 2 * cRb[c] / kLatUnit;
 }
 
-__device__ inline int pair_target(const int *__restrict__ off, int nc, int64_t e) {
-  int lo = 0, hi = nc;  // largest c with off[c] <= e
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // Mark the (tile, offset) steps that have at least one pair.
-__global__ void k_lat_mark(const int *__restrict__ off, int nc, int64_t np, const int *__restrict__ pkey,
+__global__ void k_lat_mark(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
                            const int *__restrict__ o_of_key, const int *__restrict__ tile_of, int noff,
                            int *__restrict__ act) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -265,7 +255,7 @@ __global__ void k_lat_mark(const int *__restrict__ off, int nc, int64_t np, cons
   const int key = pkey[e];
   const int o = key >= 0 ? o_of_key[key] : -1;
   if (o < 0) return;
-  const int tile = tile_of[pair_target(off, nc, e)];
+  const int tile = tile_of[ptgt[e]];
   if (tile >= 0) act[(size_t)tile * noff + o] = 1;
 }
 
@@ -276,23 +266,50 @@ __global__ void k_lat_alist(const int *__restrict__ act, const int *__restrict__
   if (i < n && act[i]) alist[aidx[i]] = (int)(i % noff);
 }
 
-// Lattice pairs: their column in their step's slot row; the rest: irregular.
-__global__ void k_lat_slots(const int *__restrict__ off, int nc, const int *__restrict__ src, int64_t np,
-                            const int *__restrict__ pkey, const int *__restrict__ o_of_key,
-                            const int *__restrict__ tile_of, const int *__restrict__ col_of, int noff,
-                            const int *__restrict__ aidx, int *__restrict__ slot, int *__restrict__ irr) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= np) return;
+// Lattice pairs claim their (step, column) slot in three passes of plain
+// stores: (1) every lattice pair writes its own index into its slot, (2) a
+// pair whose slot holds another index (a repeated source: same target and
+// offset) and every non-lattice pair is irregular, counted per wave into
+// *nirr, (3) the slots' pair indices become source cells.
+__device__ inline int64_t lat_slot_of(int64_t e, const int *__restrict__ ptgt, const int *__restrict__ pkey,
+                                      const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
+                                      const int *__restrict__ col_of, int noff, const int *__restrict__ aidx) {
   const int key = pkey[e];
   const int o = key >= 0 ? o_of_key[key] : -1;
-  const int t = pair_target(off, nc, e);
+  const int t = ptgt[e];
   const int tile = tile_of[t];
-  // A second pair at an occupied slot (a repeated source) stays per pair.
-  if (o >= 0 && tile >= 0 &&
-      atomicCAS(&slot[(size_t)aidx[(size_t)tile * noff + o] * 64 + col_of[t]], -1, src[e]) == -1)
-    irr[e] = 0;
-  else
-    irr[e] = 1;
+  if (o < 0 || tile < 0) return -1;
+  return (int64_t)aidx[(size_t)tile * noff + o] * 64 + col_of[t];
+}
+
+__global__ void k_lat_slots(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
+                            const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
+                            const int *__restrict__ col_of, int noff, const int *__restrict__ aidx,
+                            int *__restrict__ slot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  const int64_t k = lat_slot_of(e, ptgt, pkey, o_of_key, tile_of, col_of, noff, aidx);
+  if (k >= 0) slot[k] = (int)e;
+}
+
+__global__ void k_lat_slots_check(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
+                                  const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
+                                  const int *__restrict__ col_of, int noff, const int *__restrict__ aidx,
+                                  const int *__restrict__ slot, int *__restrict__ irr, int *__restrict__ nirr) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ir = false;
+  if (e < np) {
+    const int64_t k = lat_slot_of(e, ptgt, pkey, o_of_key, tile_of, col_of, noff, aidx);
+    ir = k < 0 || slot[k] != (int)e;
+    irr[e] = ir ? 1 : 0;
+  }
+  const unsigned long long m = __ballot(ir);
+  if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(nirr, (int)__popcll(m));
+}
+
+__global__ void k_lat_slots_src(int64_t n, const int *__restrict__ src, int *__restrict__ slot) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && slot[i] >= 0) slot[i] = src[slot[i]];
 }
 
 // Offsets present in the list, numbered in key order.
@@ -513,14 +530,27 @@ static void lattice_run(Handle &h, M2LList &irr) {
   const int64_t np = h.nm2l;
   const int TPB = 256;
   const int gp = div_up(np, TPB);
-  // 1. Classify pairs and histogram their offsets.
+  const bool f32 = h.cfg.precision == 1;
+  // 0. Source multipoles scaled per cell (independent of the pair lists).
+  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
+  if (f32) {
+    h.lat_Msf.resize(nms);
+    hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
+                       h.lat_Msf.p);
+  } else {
+    h.lat_Ms.resize(nms);
+    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
+                       h.lat_Ms.p);
+  }
+  // 1. Classify pairs (offset key, target cell) and flag the offsets used.
   h.lat_pkey.resize(np > 0 ? np : 1);
+  h.lat_ptgt.resize(np > 0 ? np : 1);
   h.lat_hist.resize(kLatKeys);
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, (int64_t)kLatKeys,
                      0);
   hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.m2l_src.p, h.m2l_img.p, np,
-                     h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_hist.p);
-  // 2. Offsets with enough pairs, numbered in key order.
+                     h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_ptgt.p, h.lat_hist.p);
+  // 2. Offsets present, numbered in key order.
   h.lat_flag.resize(kLatKeys + 1);
   h.lat_kscan.resize(kLatKeys + 1);
   h.lat_ookey.resize(kLatKeys);
@@ -547,6 +577,16 @@ static void lattice_run(Handle &h, M2LList &irr) {
   EXA_HIP(hipMemcpyAsync(ghist, h.lat_thist.p, sizeof(ghist), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipMemcpyAsync(&noff, h.lat_kscan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
+  // T(d) of every offset: runs while the host lays out the tiles and steps.
+  if (noff > 0) {
+    if (f32) {
+      h.lat_Tf.resize((size_t)noff * TPad<P, float>::TEP);
+      hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_Tf.p);
+    } else {
+      h.lat_T.resize((size_t)noff * TPad<P, double>::TEP);
+      hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_T.p);
+    }
+  }
   int gb[512], ntiles = 0, start = 0;
   for (int g = 0; g < 256; g++) {
     gb[g] = start;
@@ -572,8 +612,8 @@ static void lattice_run(Handle &h, M2LList &irr) {
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(nsteps_all + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p,
                      nsteps_all + 1, 0);
   if (nsteps_all)
-    hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, np, h.lat_pkey.p,
-                       h.lat_ookey.p, h.lat_tile_of.p, noff, h.lat_act.p);
+    hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
+                       h.lat_tile_of.p, noff, h.lat_act.p);
   exclusive_scan_int(h, h.lat_act.p, h.lat_aidx.p, (int)(nsteps_all + 1));
   int nact = 0;
   EXA_HIP(hipMemcpyAsync(&nact, h.lat_aidx.p + nsteps_all, sizeof(int), hipMemcpyDeviceToHost, h.stream));
@@ -587,25 +627,34 @@ static void lattice_run(Handle &h, M2LList &irr) {
     EXA_HIP(hipMemcpy2DAsync(h.lat_step0.p, sizeof(int), h.lat_aidx.p, (size_t)noff * sizeof(int), sizeof(int),
                              ntiles + 1, hipMemcpyDeviceToDevice, h.stream));
   }
+  // 5. Slots of the lattice pairs; the rest (and repeats) are irregular.
   const int64_t nslot = (int64_t)nact * 64;
   h.lat_slot.resize(nslot > 0 ? nslot : 1);
   if (nslot)
     hipLaunchKernelGGL(k_fill_int, dim3(div_up(nslot, TPB)), dim3(TPB), 0, h.stream, h.lat_slot.p, nslot, -1);
   h.lat_irr.resize(np + 1);
-  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.m2l_src.p, np, h.lat_pkey.p,
-                     h.lat_ookey.p, h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p, h.lat_irr.p);
-  // 5. The irregular pairs' CSR.
-  h.lat_scan.resize(np + 1);
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_irr.p + np, (int64_t)1, 0);
-  exclusive_scan_int(h, h.lat_irr.p, h.lat_scan.p, (int)(np + 1));
-  h.irr_off.resize(nc + 1);
-  hipLaunchKernelGGL(k_irr_csr, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.lat_scan.p,
-                     h.irr_off.p);
+  h.lat_cnt.resize(1);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_cnt.p, (int64_t)1, 0);
+  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
+                     h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p);
+  hipLaunchKernelGGL(k_lat_slots_check, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p,
+                     h.lat_ookey.p, h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p, h.lat_irr.p,
+                     h.lat_cnt.p);
+  if (nslot)
+    hipLaunchKernelGGL(k_lat_slots_src, dim3(div_up(nslot, TPB)), dim3(TPB), 0, h.stream, nslot, h.m2l_src.p,
+                       h.lat_slot.p);
   int nirr = 0;
-  EXA_HIP(hipMemcpyAsync(&nirr, h.lat_scan.p + np, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(&nirr, h.lat_cnt.p, sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
+  // 6. The irregular pairs' CSR (empty for a uniform free-space tree).
   int nitl = 0;
+  h.irr_off.resize(nc + 1);
   if (nirr) {
+    h.lat_scan.resize(np + 1);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_irr.p + np, (int64_t)1, 0);
+    exclusive_scan_int(h, h.lat_irr.p, h.lat_scan.p, (int)(np + 1));
+    hipLaunchKernelGGL(k_irr_csr, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.lat_scan.p,
+                       h.irr_off.p);
     h.irr_src.resize(nirr);
     h.irr_img.resize(nirr);
     hipLaunchKernelGGL(k_irr_compact, dim3(gp), dim3(TPB), 0, h.stream, h.lat_irr.p, h.lat_scan.p, np, h.m2l_src.p,
@@ -624,33 +673,21 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_stats_pairs = np - nirr;
   h.lat_stats_noff = noff;
   h.lat_stats_tiles = ntiles;
-  // 6. Matrices, the contraction and the reduction.
+  // 7. The contraction (timed by ev[12..13]) and the reduction.
   if (noff == 0 || ntiles == 0 || nact == 0) return;
   h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
+  const dim3 grid(ntiles * kLatSeg);
   EXA_HIP(hipEventRecord(h.ev[12], h.stream));
   h.lat_timed = true;
-  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
-  const dim3 grid(ntiles * kLatSeg);
-  if (h.cfg.precision == 1) {
-    h.lat_Tf.resize((size_t)noff * TPad<P, float>::TEP);
-    h.lat_Msf.resize(nms);
-    hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_Tf.p);
-    hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
-                       h.lat_Msf.p);
+  if (f32)
     hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
                        h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
-  } else {
-    h.lat_T.resize((size_t)noff * TPad<P, double>::TEP);
-    h.lat_Ms.resize(nms);
-    hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_T.p);
-    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
-                       h.lat_Ms.p);
+  else
     hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
                        h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
-  }
+  EXA_HIP(hipEventRecord(h.ev[13], h.stream));
   hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.lat_tile_w.p,
                      h.lat_part.p, h.L.p);
-  EXA_HIP(hipEventRecord(h.ev[13], h.stream));
   EXA_HIP(hipGetLastError());
 }
 
