@@ -132,6 +132,7 @@ struct Handle {
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
   DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
+  bool lat_ms_ready = false;     // lat_Ms / lat_Msf already queued for this evaluation
   DBuf<double> lat_tile_w, lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;
   int64_t lat_stats_pairs = 0;
@@ -182,6 +183,7 @@ void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const do
 void run_mass_normalise(Handle &h);
 void run_ropt(Handle &h);
 bool run_m2l_lattice(Handle &h, M2LList &irr);
+void run_lattice_mscale(Handle &h, hipStream_t st);
 bool updown_mfma_enabled(const Handle &h);
 void run_m2m_level_mfma(Handle &h, const int *cells, int cnt, hipStream_t st);
 void run_l2l_level_mfma(Handle &h, const int *cells, int cnt);

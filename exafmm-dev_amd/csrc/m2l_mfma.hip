@@ -524,6 +524,22 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 template <int P>
+static void mscale_impl(Handle &h, hipStream_t st) {
+  using S = MfmaShape<P>;
+  const int nc = h.ncells;
+  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
+  if (h.cfg.precision == 1) {
+    h.lat_Msf.resize(nms);
+    hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, 256)), dim3(256), 0, st, nc, h.cRb.p, h.M.p,
+                       h.lat_Msf.p);
+  } else {
+    h.lat_Ms.resize(nms);
+    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, 256)), dim3(256), 0, st, nc, h.cRb.p, h.M.p,
+                       h.lat_Ms.p);
+  }
+}
+
+template <int P>
 static void lattice_run(Handle &h, M2LList &irr) {
   using S = MfmaShape<P>;
   const int nc = h.ncells;
@@ -531,17 +547,10 @@ static void lattice_run(Handle &h, M2LList &irr) {
   const int TPB = 256;
   const int gp = div_up(np, TPB);
   const bool f32 = h.cfg.precision == 1;
-  // 0. Source multipoles scaled per cell (independent of the pair lists).
-  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
-  if (f32) {
-    h.lat_Msf.resize(nms);
-    hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
-                       h.lat_Msf.p);
-  } else {
-    h.lat_Ms.resize(nms);
-    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, TPB)), dim3(TPB), 0, h.stream, nc, h.cRb.p, h.M.p,
-                       h.lat_Ms.p);
-  }
+  // 0. Source multipoles scaled per cell (independent of the pair lists;
+  // already queued behind the multipoles when the upward pass was split).
+  if (!h.lat_ms_ready) mscale_impl<P>(h, h.stream);
+  h.lat_ms_ready = false;
   // 1. Classify pairs (offset key, target cell) and flag the offsets used.
   h.lat_pkey.resize(np > 0 ? np : 1);
   h.lat_ptgt.resize(np > 0 ? np : 1);
@@ -691,11 +700,33 @@ static void lattice_run(Handle &h, M2LList &irr) {
   EXA_HIP(hipGetLastError());
 }
 
+static bool lattice_applies(const Handle &h) {
+  return h.cfg.basis == 0 && !h.cfg.mass && h.lattice_m2l && h.cfg.P >= 4 && h.cfg.P <= 12 && (h.cfg.P % 2) == 0;
+}
+
+// Scaled source multipoles for the lattice contraction, queued on stream st
+// right behind the multipoles (the split upward pass).
+void run_lattice_mscale(Handle &h, hipStream_t st) {
+  if (!lattice_applies(h)) return;
+  switch (h.cfg.P) {
+    case 4: mscale_impl<4>(h, st); break;
+    case 6: mscale_impl<6>(h, st); break;
+    case 8: mscale_impl<8>(h, st); break;
+    case 10: mscale_impl<10>(h, st); break;
+    case 12: mscale_impl<12>(h, st); break;
+  }
+  EXA_HIP(hipGetLastError());
+  h.lat_ms_ready = true;
+}
+
 bool run_m2l_lattice(Handle &h, M2LList &irr) {
   h.lat_timed = false;
   h.lat_stats_pairs = 0;
   h.lat_stats_noff = h.lat_stats_tiles = 0;
-  if (h.cfg.basis != 0 || h.cfg.mass || h.nm2l == 0 || !h.lattice_m2l) return false;
+  if (!lattice_applies(h) || h.nm2l == 0) {
+    h.lat_ms_ready = false;
+    return false;
+  }
   switch (h.cfg.P) {
     case 4: lattice_run<4>(h, irr); return true;
     case 6: lattice_run<6>(h, irr); return true;

@@ -366,6 +366,7 @@ void run_upward_split(Handle &h) {
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[16], 0));
   EXA_HIP(hipEventRecord(h.ev[17], h.stream2));
   EXA_DISPATCH_P3(upward_impl, h, 1, h.stream2);
+  run_lattice_mscale(h, h.stream2);  // M2L input, ahead of P2P on this stream
   EXA_HIP(hipEventRecord(h.ev_mult, h.stream2));
   EXA_DISPATCH_P3(upward_impl, h, 2, h.stream);
 }
