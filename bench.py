@@ -71,6 +71,24 @@ def cpu_baseline(n, P):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines `traffic <kernel> <bytes>`
+    in profiles/*.md), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, name)) as fh:
+            for line in fh:
+                m = re.match(r"\s*traffic\s+(\S+)\s+(\d+)\s*$", line)
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
+
+
 def copy_probe():
     """One device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB): the
     known-size kernel that scales the HBM byte counters."""
@@ -160,7 +178,8 @@ def main():
     w, t, bound, peak = kern[dom_name]
     ach = w / (t * 1e-3) / 1e12
     roofline = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 5), "traffic": None, "kernel": dom_name, "kernel_ms": round(t, 4)}
+                "frac": round(ach / peak, 5), "traffic": measured_traffic(dom_name) if n == 1 << 20 and P == 10 else None,
+                "kernel": dom_name, "kernel_ms": round(t, 4)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, P)
     line = {
