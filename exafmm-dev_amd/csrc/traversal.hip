@@ -52,21 +52,9 @@ __device__ inline int classify(const CellView &cv, int ci, int cj, int img, doub
 // stands for both directions: M2L and P2P emit both directed entries, and a
 // cell split against itself visits only the child pairs (a, b), a <= b, as
 // TraverseRange skips the (latter, former) quadrant (traversal.h:108-120).
-// Counts per frontier pair (M2L entries, P2P entries, next-frontier pairs)
-// packed in one int4 so that a single exclusive scan gives all three
-// output offsets.
-__global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
-                           int *__restrict__ act, int4 *__restrict__ cnt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nf) return;
-  if (i == nf) {  // trailing zero so that the exclusive scan ends in the totals
-    cnt[i] = make_int4(0, 0, 0, 0);
-    return;
-  }
-  const int4 f = fr[i];
-  const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
+// Output counts of one frontier pair: (M2L entries, P2P entries, next pairs).
+__device__ inline int4 pair_counts(const CellView &cv, int4 f, int a, int mutual) {
   const int both = (mutual && f.x != f.y) ? 2 : 1;
-  act[i] = a;
   int nx = 0;
   if (a == 2) {
     nx = cv.nchild[f.x];
@@ -76,20 +64,75 @@ __global__ void k_classify(CellView cv, const int4 *__restrict__ fr, int nf, dou
     const int nci = cv.nchild[f.x];
     nx = (mutual && f.x == f.y) ? nci * (nci + 1) / 2 : nci * cv.nchild[f.y];
   }
-  cnt[i] = make_int4(a == 0 ? both : 0, a == 1 ? both : 0, nx, 0);
+  return make_int4(a == 0 ? both : 0, a == 1 ? both : 0, nx, 0);
 }
 
-__global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
-                       const int4 *__restrict__ ofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
-                       unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
-                       unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline int4 add4(int4 a, int4 b) { return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+constexpr int kTravBlock = 256;
+
+// Exclusive prefix of v over the 256 threads of the block (wave scans, then
+// the wave totals); *total receives the block sum.
+__device__ inline int4 block_exscan4(int4 v, int4 *total) {
+  __shared__ int4 wsum[kTravBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int4 inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int4 t = make_int4(__shfl_up(inc.x, d), __shfl_up(inc.y, d), __shfl_up(inc.z, d), 0);
+    if (lane >= d) inc = add4(inc, t);
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int4 base = make_int4(0, 0, 0, 0), tot = make_int4(0, 0, 0, 0);
+  for (int k = 0; k < kTravBlock / 64; k++) {
+    if (k < w) base = add4(base, wsum[k]);
+    tot = add4(tot, wsum[k]);
+  }
+  __syncthreads();
+  *total = tot;
+  return make_int4(base.x + inc.x - v.x, base.y + inc.y - v.y, base.z + inc.z - v.z, 0);
+}
+
+// Pass 1 over the frontier: action per pair and the block's output totals.
+__global__ void __launch_bounds__(kTravBlock)
+    k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
+               int *__restrict__ act, int4 *__restrict__ blk) {
+  const int i = blockIdx.x * kTravBlock + threadIdx.x;
+  int4 c = make_int4(0, 0, 0, 0);
+  if (i < nf) {
+    const int4 f = fr[i];
+    const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
+    act[i] = a;
+    c = pair_counts(cv, f, a, mutual);
+  }
+  int4 tot;
+  block_exscan4(c, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) blk[gridDim.x] = make_int4(0, 0, 0, 0);  // the scan ends in the totals
+}
+
+// Pass 2: each pair's output offsets = its block's offset (scan of the block
+// totals) + its prefix inside the block; the same order as one global
+// exclusive scan, so the lists are deterministic.
+__global__ void __launch_bounds__(kTravBlock)
+    k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
+           const int4 *__restrict__ blkofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i,
+           int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+           int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
+  const int i = blockIdx.x * kTravBlock + threadIdx.x;
+  int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
+  int a = -1;
+  if (i < nf) {
+    f = fr[i];
+    a = act[i];
+    c = pair_counts(cv, f, a, mutual);
+  }
+  int4 tot;
+  const int4 ofs = add4(block_exscan4(c, &tot), blkofs[blockIdx.x]);
   if (i >= nf) return;
-  const int4 f = fr[i];
-  const int a = act[i];
   const bool both = mutual && f.x != f.y;
   if (a == 0) {
-    const int64_t o = base_m2l + ofs[i].x;
+    const int64_t o = base_m2l + ofs.x;
     m2l_i[o] = f.x;
     m2l_j[o] = f.y;
     m2l_img[o] = (unsigned char)f.z;
@@ -99,7 +142,7 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mut
       m2l_img[o + 1] = (unsigned char)f.z;
     }
   } else if (a == 1) {
-    const int64_t o = base_p2p + ofs[i].y;
+    const int64_t o = base_p2p + ofs.y;
     p2p_i[o] = f.x;
     p2p_j[o] = f.y;
     p2p_img[o] = (unsigned char)f.z;
@@ -109,7 +152,7 @@ __global__ void k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mut
       p2p_img[o + 1] = (unsigned char)f.z;
     }
   } else {
-    int o = ofs[i].z;
+    int o = ofs.z;
     if (a == 2) {
       const int c0 = cv.child[f.x], nc = cv.nchild[f.x];
       for (int c = 0; c < nc; c++) next[o++] = make_int4(c0 + c, f.y, f.z, 0);
@@ -257,16 +300,17 @@ void run_lists(Handle &h) {
   grow_keep(h, pimg, (size_t)h.ncells * 48 + 1024, 0);
   h.cnt.reserve(4);
   while (nf > 0) {
+    const int nb = div_up(nf, kTravBlock);
     h.ia.reserve(nf + 1);  // action
-    h.ib.reserve(4 * ((size_t)nf + 1));  // counts (int4)
-    h.ic.reserve(4 * ((size_t)nf + 1));  // offsets (int4)
+    h.ib.reserve(4 * ((size_t)nb + 1));  // block totals (int4)
+    h.ic.reserve(4 * ((size_t)nb + 1));  // block offsets (int4)
     int *act = h.ia.p;
-    int4 *cnt = reinterpret_cast<int4 *>(h.ib.p), *ofs = reinterpret_cast<int4 *>(h.ic.p);
-    hipLaunchKernelGGL(k_classify, dim3(div_up(nf + 1, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
-                       h.cfg.nspawn, mutual, act, cnt);
-    exclusive_scan_int4(h, cnt, ofs, nf + 1);
+    int4 *blk = reinterpret_cast<int4 *>(h.ib.p), *blkofs = reinterpret_cast<int4 *>(h.ic.p);
+    hipLaunchKernelGGL(k_classify, dim3(nb), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
+                       h.cfg.nspawn, mutual, act, blk);
+    exclusive_scan_int4(h, blk, blkofs, nb + 1);
     int4 tot4;
-    EXA_HIP(hipMemcpyAsync(&tot4, ofs + nf, sizeof(tot4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipMemcpyAsync(&tot4, blkofs + nb, sizeof(tot4), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
     const int tot[3] = {tot4.x, tot4.y, tot4.z};
     grow_keep(h, mi, nm + tot[0] + 1, nm);
@@ -276,7 +320,7 @@ void run_lists(Handle &h) {
     grow_keep(h, pj, np + tot[1] + 1, np);
     grow_keep(h, pimg, np + tot[1] + 1, np);
     h.fb.reserve((size_t)tot[2] + 1);
-    hipLaunchKernelGGL(k_emit, dim3(div_up(nf, T)), dim3(T), 0, h.stream, cv, h.fa.p, nf, mutual, act, ofs, nm, np,
+    hipLaunchKernelGGL(k_emit, dim3(nb), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, nf, mutual, act, blkofs, nm, np,
                        mi.p, mj.p, mimg.p, pi.p, pj.p, pimg.p, h.fb.p);
     nm += tot[0];
     np += tot[1];
