@@ -2,6 +2,7 @@
 // point converts exceptions into status codes and keeps the message on the
 // handle.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -72,7 +73,9 @@ static void set_device(Handle &h) { EXA_HIP(hipSetDevice(h.cfg.device)); }
 static void do_build(Handle &h) {
   ensure(h.have_bounds, "build_tree: bounds not computed");
   ensure(h.n > 0, "build_tree: no bodies");
-  if (h.cfg.builder == 0)
+  if (h.nsys > 1)
+    run_build_forest(h);
+  else if (h.cfg.builder == 0)
     run_build_uniform(h);
   else
     run_build_adaptive(h);
@@ -206,6 +209,7 @@ int exafmm_hip_set_bodies(exafmm_hip_handle *H, int n, const double *xyzq) {
   return guarded(H, [&](Handle &h) {
     ensure(n >= 0 && (n == 0 || xyzq), "set_bodies: bad arguments");
     h.n = n;
+    h.nsys = 1;
     h.in4.resize(n > 0 ? n : 1);
     if (n) EXA_HIP(hipMemcpyAsync(h.in4.p, xyzq, n * sizeof(double4), hipMemcpyHostToDevice, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
@@ -217,6 +221,7 @@ int exafmm_hip_set_bodies_device(exafmm_hip_handle *H, int n, const double *d_xy
   return guarded(H, [&](Handle &h) {
     ensure(n >= 0 && (n == 0 || d_xyzq), "set_bodies_device: bad arguments");
     h.n = n;
+    h.nsys = 1;
     h.in4.resize(n > 0 ? n : 1);
     if (n) EXA_HIP(hipMemcpyAsync(h.in4.p, d_xyzq, n * sizeof(double4), hipMemcpyDeviceToDevice, h.stream));
     h.have_bounds = h.have_tree = h.have_up = h.have_trav = false;
@@ -258,12 +263,20 @@ int exafmm_hip_downward_pass(exafmm_hip_handle *H) {
   });
 }
 
-int exafmm_hip_evaluate(exafmm_hip_handle *H) {
-  return guarded(H, [&](Handle &h) {
+static void evaluate_all(Handle &h) {
+  {
     EXA_HIP(hipEventRecord(h.ev[0], h.stream));
-    run_bounds(h);
-    EXA_HIP(hipEventRecord(h.ev[1], h.stream));
-    do_build(h);
+    if (h.nsys > 1) {  // a forest: per-system bounds inside the builder
+      h.have_bounds = true;
+      EXA_HIP(hipEventRecord(h.ev[1], h.stream));
+      run_build_forest(h);
+      h.have_tree = true;
+      h.have_up = h.have_trav = false;
+    } else {
+      run_bounds(h);
+      EXA_HIP(hipEventRecord(h.ev[1], h.stream));
+      do_build(h);
+    }
     EXA_HIP(hipEventRecord(h.ev[2], h.stream));
     // Inside one evaluation the multipoles (P2M, M2M) run on the second
     // stream while the main stream computes radii and builds the lists.
@@ -294,6 +307,39 @@ int exafmm_hip_evaluate(exafmm_hip_handle *H) {
     s.t_total = elapsed(h.ev[0], h.ev[8]);
     s.t_m2l_mfma = h.lat_timed ? elapsed(h.ev[12], h.ev[13]) : 0;
     s.t_m2l_pairs = h.pair_timed ? elapsed(h.ev[14], h.ev[15]) : 0;
+  }
+}
+
+int exafmm_hip_evaluate(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) { evaluate_all(h); });
+}
+
+int exafmm_hip_evaluate_batch(exafmm_hip_handle *H, int nsys, const int *n, const double *xyzq, double *trg) {
+  return guarded(H, [&](Handle &h) {
+    ensure(nsys >= 1 && n && xyzq && trg, "evaluate_batch: bad arguments");
+    ensure(h.cfg.builder == 0 && h.cfg.images == 0 && h.cfg.basis == 0,
+           "evaluate_batch: uniform builder, no periodic images, spherical basis");
+    std::vector<int> off(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+      ensure(n[s] >= 1, "evaluate_batch: every system needs at least one body");
+      ensure((int64_t)off[s] + n[s] <= INT32_MAX, "evaluate_batch: too many bodies");
+      off[s + 1] = off[s] + n[s];
+    }
+    const int total = off[nsys];
+    h.n = total;
+    h.in4.resize(total);
+    EXA_HIP(hipMemcpyAsync(h.in4.p, xyzq, (size_t)total * sizeof(double4), hipMemcpyHostToDevice, h.stream));
+    h.nsys = nsys;
+    h.sys_off = off;
+    h.have_bounds = h.have_tree = h.have_up = h.have_trav = false;
+    evaluate_all(h);
+    DBuf<double4> o;
+    o.resize(total);
+    hipLaunchKernelGGL(k_scatter_results, dim3(div_up(total, 256)), dim3(256), 0, h.stream, h.trg4.p, h.bidx.p, total,
+                       o.p);
+    EXA_HIP(hipGetLastError());
+    EXA_HIP(hipMemcpyAsync(trg, o.p, (size_t)total * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
   });
 }
 

@@ -79,6 +79,12 @@ struct Handle {
 
   // Bodies.
   int n = 0;
+  // Batched evaluation: nsys > 1 independent systems, system s owning bodies
+  // [sys_off[s], sys_off[s+1]) of the input; the tree is a forest.
+  int nsys = 1;
+  std::vector<int> sys_off;
+  DBuf<int> sys_offd;
+  DBuf<double> sys_par;  // per-system builder parameters
   DBuf<double4> in4;     // caller order (x, y, z, q)
   DBuf<double4> body4;   // tree order
   DBuf<int> bidx;        // tree order -> caller index (Body::IBODY)
@@ -133,7 +139,7 @@ struct Handle {
   DBuf<unsigned char> irr_img;
   DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
   bool lat_ms_ready = false;     // lat_Ms / lat_Msf already queued for this evaluation
-  DBuf<double> lat_tile_w, lat_T, lat_Ms;
+  DBuf<double> lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;
   int64_t lat_stats_pairs = 0;
   int lat_stats_noff = 0, lat_stats_tiles = 0;
@@ -171,6 +177,7 @@ struct M2LList {
 // Phase implementations (one translation unit each).
 void run_bounds(Handle &h);
 void run_build_uniform(Handle &h);
+void run_build_forest(Handle &h);
 void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
 void run_upward_split(Handle &h);

@@ -229,8 +229,8 @@ __global__ void k_tile_hist(int nc, const int *__restrict__ key, int *__restrict
 
 // gb[0..255] = group start in sorted order, gb[256..511] = first tile of the group.
 __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *__restrict__ sval,
-                              const int *__restrict__ gb, const double *__restrict__ cRb, int *__restrict__ tile_tgt,
-                              int *__restrict__ tile_of, int *__restrict__ col_of, double *__restrict__ tile_u) {
+                              const int *__restrict__ gb, int *__restrict__ tile_tgt, int *__restrict__ tile_of,
+                              int *__restrict__ col_of) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nc) return;
   const int g = skey[p], c = sval[p];
@@ -243,7 +243,6 @@ __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *_
   tile_tgt[t * 64 + col] = c;
   tile_of[c] = t;
   col_of[c] = col;
-  if (col == 0) tile_u[t] = 2 * cRb[c] / kLatUnit;
 }
 
 // Mark the (tile, offset) steps that have at least one pair.
@@ -499,16 +498,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
 }
 
 // L of each tile column += u^-(j+1) * (sum of the segments' partials, in
-// segment order).
+// segment order), u = w / kLatUnit of the column's own target cell (the cells
+// of a tile share a level, not necessarily a size: batched systems).
 template <int P>
 __global__ void __launch_bounds__(256)
-    k_lat_reduce(const int *__restrict__ tile_tgt, const double *__restrict__ tile_u,
-                 const double *__restrict__ part, cplx *__restrict__ L) {
+    k_lat_reduce(const int *__restrict__ tile_tgt, const double *__restrict__ cRb, const double *__restrict__ part,
+                 cplx *__restrict__ L) {
   using S = MfmaShape<P>;
   constexpr int NR = S::NR, NRP = S::RB * 16;
   constexpr DegreeTable<P> deg{};
   const int tile = blockIdx.x;
-  const double iw = 1.0 / tile_u[tile];
   double *Lr = reinterpret_cast<double *>(L);
   for (int i = threadIdx.x; i < NR * 64; i += 256) {
     const int row = i / 64, col = i % 64;
@@ -516,6 +515,7 @@ __global__ void __launch_bounds__(256)
     if (tgt < 0) continue;
     double v = 0;
     for (int sg = 0; sg < kLatSeg; sg++) v += part[((size_t)(tile * kLatSeg + sg) * NRP + row) * 64 + col];
+    const double iw = 1.0 / (2 * cRb[tgt] / kLatUnit);
     double f = iw;
     for (int q = 0; q < deg.n[row >> 1]; q++) f *= iw;
     Lr[(size_t)tgt * NR + row] += v * f;
@@ -606,14 +606,13 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_gb.resize(512);
   EXA_HIP(hipMemcpyAsync(h.lat_gb.p, gb, sizeof(gb), hipMemcpyHostToDevice, h.stream));
   h.lat_tile_tgt.resize((size_t)(ntiles > 0 ? ntiles : 1) * 64);
-  h.lat_tile_w.resize(ntiles > 0 ? ntiles : 1);
   h.lat_tile_of.resize(nc);
   h.lat_col_of.resize(nc);
   if (ntiles)
     hipLaunchKernelGGL(k_fill_int, dim3(div_up((int64_t)ntiles * 64, TPB)), dim3(TPB), 0, h.stream, h.lat_tile_tgt.p,
                        (int64_t)ntiles * 64, -1);
   hipLaunchKernelGGL(k_tile_assign, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, sval, h.lat_gb.p,
-                     h.cRb.p, h.lat_tile_tgt.p, h.lat_tile_of.p, h.lat_col_of.p, h.lat_tile_w.p);
+                     h.lat_tile_tgt.p, h.lat_tile_of.p, h.lat_col_of.p);
   // 4. Steps: (tile, offset) with at least one pair, compacted per tile.
   const int64_t nsteps_all = (int64_t)ntiles * noff;
   h.lat_act.resize(nsteps_all + 1);
@@ -695,7 +694,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
     hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
                        h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
   EXA_HIP(hipEventRecord(h.ev[13], h.stream));
-  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.lat_tile_w.p,
+  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
                      h.lat_part.p, h.L.p);
   EXA_HIP(hipGetLastError());
 }

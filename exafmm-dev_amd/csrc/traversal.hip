@@ -182,6 +182,13 @@ __global__ void k_init_frontier(int4 *__restrict__ fr, int images) {
   }
 }
 
+// A forest (batched systems): every root against itself, no images; the
+// traversal never pairs cells of two systems.
+__global__ void k_init_frontier_roots(int4 *__restrict__ fr, const int *__restrict__ roots, int nroots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nroots) fr[i] = make_int4(roots[i], roots[i], 13, 0);
+}
+
 // CSR offsets from the target-sorted keys: off[c] = first index with key >= c.
 __global__ void k_csr_offsets(const int *__restrict__ keys, int64_t n, int nc, int *__restrict__ off) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,8 +295,15 @@ void run_lists(Handle &h) {
   int nf = h.cfg.images == 0 ? 1 : 27;
   // Periodic images are traversed non-mutually (traversal.h:250-259).
   const int mutual = (h.cfg.mutual && h.cfg.images == 0) ? 1 : 0;
-  h.fa.reserve(64);
-  hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
+  if (h.nsys > 1) {
+    nf = h.lvl_off[1] - h.lvl_off[0];  // the roots: level 0 of the forest
+    h.fa.reserve(nf + 64);
+    hipLaunchKernelGGL(k_init_frontier_roots, dim3(div_up(nf, 256)), dim3(256), 0, h.stream, h.fa.p,
+                       h.lvl_cells.p + h.lvl_off[0], nf);
+  } else {
+    h.fa.reserve(64);
+    hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
+  }
   // Start the raw lists at a capacity that covers uniform trees without
   // regrowth (about 180 M2L and 65 P2P sources per cell at theta = 0.4).
   grow_keep(h, mi, (size_t)h.ncells * 128 + 1024, 0);

@@ -322,4 +322,282 @@ void run_build_uniform(Handle &h) {
   EXA_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Forest of uniform trees (batched evaluation of independent systems).
+// System s owns bodies [off[s], off[s+1]) of the input and gets its own
+// bounds, depth L_s and Morton keys exactly as a single-system build of its
+// bodies would (the same float/int arithmetic); the bodies are sorted by
+// (s, key_s), so systems stay contiguous and in input order.  Level t holds
+// the level-t cells of every system with L_s >= t; a system's cells are the
+// ones its single-system build makes, in the same relative order.
+
+// Per-system min/max, one workgroup per system.
+__global__ void __launch_bounds__(256)
+    k_bounds_seg(const double4 *__restrict__ b, const int *__restrict__ off, double *__restrict__ out) {
+  __shared__ double s[6][256];
+  const int sys = blockIdx.x, i0 = off[sys], i1 = off[sys + 1];
+  double mn[3] = {b[i0].x, b[i0].y, b[i0].z};
+  double mx[3] = {mn[0], mn[1], mn[2]};
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double4 v = b[i];
+    mn[0] = v.x < mn[0] ? v.x : mn[0];
+    mn[1] = v.y < mn[1] ? v.y : mn[1];
+    mn[2] = v.z < mn[2] ? v.z : mn[2];
+    mx[0] = v.x > mx[0] ? v.x : mx[0];
+    mx[1] = v.y > mx[1] ? v.y : mx[1];
+    mx[2] = v.z > mx[2] ? v.z : mx[2];
+  }
+  for (int d = 0; d < 3; d++) {
+    s[d][threadIdx.x] = mn[d];
+    s[3 + d][threadIdx.x] = mx[d];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) {
+        const double a = s[d][threadIdx.x + w], c = s[3 + d][threadIdx.x + w];
+        s[d][threadIdx.x] = a < s[d][threadIdx.x] ? a : s[d][threadIdx.x];
+        s[3 + d][threadIdx.x] = c > s[3 + d][threadIdx.x] ? c : s[3 + d][threadIdx.x];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[6 * sys + threadIdx.x] = s[threadIdx.x][0];
+}
+
+struct ForestSys {
+  double x0, y0, z0;  // bounds minimum (the key origin)
+  float d;            // leaf cell size, 2 R0 / 2^L
+  int L;              // depth
+  int begin, end;     // body range
+};
+
+// System of every body (input order), by binary search over the offsets.
+__global__ void k_sys_of(const int *__restrict__ off, int nsys, int n, int *__restrict__ sys) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nsys;  // largest s with off[s] <= i
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  sys[i] = lo;
+}
+
+// Sort key (s << 32) | key_s, key_s from getKey at the system's depth.
+__global__ void k_morton_forest(const double4 *__restrict__ in, int n, const int *__restrict__ sys,
+                                const ForestSys *__restrict__ fs, uint64_t *__restrict__ key, int *__restrict__ idx) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int s = sys[b];
+  const ForestSys f = fs[s];
+  const double4 v = in[b];
+  const double dd = (double)f.d;
+  int ix = (int)((v.x - f.x0) / dd);
+  int iy = (int)((v.y - f.y0) / dd);
+  int iz = (int)((v.z - f.z0) / dd);
+  int id = 0;
+  for (int l = 0; l != f.L; ++l) {
+    id += (ix & 1) << (3 * l);
+    id += (iy & 1) << (3 * l + 1);
+    id += (iz & 1) << (3 * l + 2);
+    ix >>= 1;
+    iy >>= 1;
+    iz >>= 1;
+  }
+  key[b] = ((uint64_t)s << 32) | (uint32_t)id;
+  idx[b] = b;
+}
+
+__device__ inline bool forest_prefix(uint64_t k, const ForestSys *__restrict__ fs, int t, int &s, uint32_t &pre) {
+  s = (int)(k >> 32);
+  const int L = fs[s].L;
+  if (t > L) return false;
+  pre = (uint32_t)k >> (3 * (L - t));
+  return true;
+}
+
+__global__ void k_level_heads_forest(const uint64_t *__restrict__ key, int n, int t, const ForestSys *__restrict__ fs,
+                                     int *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int f = 0;
+  int s, s2;
+  uint32_t p, p2;
+  if (i < n && forest_prefix(key[i], fs, t, s, p))
+    f = i == 0 || !forest_prefix(key[i - 1], fs, t, s2, p2) || s2 != s || p2 != p;
+  flags[i] = f;
+}
+
+// As k_level_cells, with the system's origin and cell size, the cell's body
+// range cut at its system's end, and leaves wherever no child heads follow.
+__global__ void k_level_cells_forest(LevelArgs a, const double4 *__restrict__ body, const uint64_t *__restrict__ key,
+                                     int n, int nc, const ForestSys *__restrict__ fs, double4 *__restrict__ cX,
+                                     double *__restrict__ cRb, int *__restrict__ cParent, int *__restrict__ cChild,
+                                     int *__restrict__ cNChild, int *__restrict__ cBody, int *__restrict__ cNBody,
+                                     uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.count) return;
+  const int start = a.pos[r];
+  const int sys = (int)(key[start] >> 32);
+  const ForestSys f = fs[sys];
+  int end = (r + 1 < a.count) ? a.pos[r + 1] : n;
+  end = end < f.end ? end : f.end;
+  const int c = a.off + a.count - 1 - r;
+  float dt = f.d;
+  for (int l = 0; l < f.L - a.level; l++) dt *= 2;  // leafs2cells doubles d per level (:166)
+  const double4 v = body[start];
+  const double dd = (double)dt;
+  const int ix = (int)((v.x - f.x0) / dd);
+  const int iy = (int)((v.y - f.y0) / dd);
+  const int iz = (int)((v.z - f.z0) / dd);
+  double4 X;
+  X.x = dd * (ix + .5) + f.x0;
+  X.y = dd * (iy + .5) + f.y0;
+  X.z = dd * (iz + .5) + f.z0;
+  X.w = dd * .5;
+  cX[c] = X;
+  cRb[c] = X.w;
+  cBody[c] = start;
+  cNBody[c] = end - start;
+  cKey[c] = (uint64_t)(int64_t)(int)((uint32_t)key[start] >> (3 * (f.L - a.level)));
+  cLevel[c] = a.level;
+  const int nch = a.scan_dn ? a.scan_dn[end] - a.scan_dn[start] : 0;
+  if (nch == 0) {
+    cNChild[c] = 0;
+    cChild[c] = nc;
+  } else {
+    cNChild[c] = nch;
+    cChild[c] = a.off_dn + a.count_dn - a.scan_dn[end];
+  }
+  if (a.level == 0) {
+    cParent[c] = nc - 1;
+  } else {
+    const int rp = a.scan_up[start + 1] - 1;
+    cParent[c] = a.off_up + a.count_up - 1 - rp;
+  }
+}
+
+void run_build_forest(Handle &h) {
+  const int n = h.n, S = h.nsys, ncrit = h.cfg.ncrit;
+  const int T = 256;
+  const std::vector<int> &off = h.sys_off;
+  h.sys_offd.resize(S + 1);
+  EXA_HIP(hipMemcpyAsync(h.sys_offd.p, off.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice, h.stream));
+  h.red.reserve(6 * (size_t)S);
+  hipLaunchKernelGGL(k_bounds_seg, dim3(S), dim3(256), 0, h.stream, h.in4.p, h.sys_offd.p, h.red.p);
+  std::vector<double> bb(6 * (size_t)S);
+  EXA_HIP(hipMemcpyAsync(bb.data(), h.red.p, bb.size() * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  std::vector<ForestSys> fs(S);
+  int Lmax = 0;
+  for (int s = 0; s < S; s++) {
+    const int ns = off[s + 1] - off[s];
+    // build_tree_omp.h:218 and bounds2box (:21-33), per system.
+    const int L = ns >= ncrit ? 1 + int(std::log((double)(ns / ncrit)) / M_LN2 / 3) : 0;
+    if (L > 10) throw Error(EXAFMM_HIP_EUNSUP, "uniform tree deeper than 10 levels overflows the reference's int keys");
+    const double *mn = &bb[6 * s], *mx = &bb[6 * s + 3];
+    double Xc[3], R0 = 0;
+    for (int d = 0; d < 3; d++) Xc[d] = (mx[d] + mn[d]) / 2;
+    for (int d = 0; d < 3; d++) {
+      R0 = std::max(Xc[d] - mn[d], R0);
+      R0 = std::max(mx[d] - Xc[d], R0);
+    }
+    R0 *= 1.00001;
+    fs[s] = ForestSys{mn[0], mn[1], mn[2], (float)(2 * R0 / (1 << L)), L, off[s], off[s + 1]};
+    Lmax = std::max(Lmax, L);
+  }
+  h.sys_par.resize(S * sizeof(ForestSys) / sizeof(double) + 1);
+  ForestSys *fsd = reinterpret_cast<ForestSys *>(h.sys_par.p);
+  EXA_HIP(hipMemcpyAsync(fsd, fs.data(), S * sizeof(ForestSys), hipMemcpyHostToDevice, h.stream));
+  h.k64a.reserve(n + 1);
+  h.k64b.reserve(n + 1);
+  h.ia.reserve(n + 1);
+  h.ib.reserve(n + 1);
+  h.ie.reserve(n + 1);
+  h.body4.resize(n);
+  h.bidx.resize(n);
+  h.trg4.resize(n);
+  hipLaunchKernelGGL(k_sys_of, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.sys_offd.p, S, n, h.ie.p);
+  hipLaunchKernelGGL(k_morton_forest, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.ie.p, fsd, h.k64a.p,
+                     h.ia.p);
+  int sbits = 1;
+  while ((1 << sbits) < S) sbits++;
+  bool alt = false;
+  sort_pairs_u64_int(h, h.k64a.p, h.k64b.p, h.ia.p, h.ib.p, n, 32 + sbits, alt);
+  uint64_t *key = alt ? h.k64b.p : h.k64a.p;
+  int *order = alt ? h.ib.p : h.ia.p;
+  hipLaunchKernelGGL(k_gather_bodies, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, order, n, h.body4.p, h.bidx.p);
+  const int L = Lmax;
+  const size_t stride = (size_t)n + 1;
+  h.ic.reserve(stride * (L + 1));
+  h.id.reserve(stride * (L + 1));
+  for (int t = 0; t <= L; t++) {
+    int *fl = h.ic.p + stride * t;
+    int *sc = h.id.p + stride * t;
+    hipLaunchKernelGGL(k_level_heads_forest, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, key, n, t, fsd, fl);
+    exclusive_scan_int(h, fl, sc, n + 1);
+  }
+  std::vector<int> counts(L + 1), tot(L + 1);
+  h.cnt.reserve(L + 1);
+  hipLaunchKernelGGL(k_strided_gather, dim3(1), dim3(64), 0, h.stream, h.id.p + n, (int64_t)stride, L + 1, h.cnt.p);
+  EXA_HIP(hipMemcpyAsync(tot.data(), h.cnt.p, (L + 1) * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  int nc = 0;
+  std::vector<int> loff(L + 2, 0);
+  for (int t = 0; t <= L; t++) {
+    counts[t] = tot[t];
+    loff[t] = nc;
+    nc += counts[t];
+  }
+  loff[L + 1] = nc;
+  h.ncells = nc;
+  h.nlevels = L + 1;
+  h.ie.reserve(std::max((size_t)nc + 1, (size_t)n + 1));
+  std::vector<int> posoff(L + 1);
+  for (int t = 0, o = 0; t <= L; t++) {
+    posoff[t] = o;
+    o += counts[t];
+  }
+  for (int t = 0; t <= L; t++)
+    hipLaunchKernelGGL(k_level_positions, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ic.p + stride * t,
+                       h.id.p + stride * t, n, h.ie.p + posoff[t]);
+  h.cX.resize(nc);
+  h.cRb.resize(nc);
+  h.cParent.resize(nc);
+  h.cChild.resize(nc);
+  h.cNChild.resize(nc);
+  h.cBody.resize(nc);
+  h.cNBody.resize(nc);
+  h.cKey.resize(nc);
+  h.cLevel.resize(nc);
+  for (int t = 0; t <= L; t++) {
+    if (counts[t] == 0) continue;
+    LevelArgs a;
+    a.pos = h.ie.p + posoff[t];
+    a.scan = h.id.p + stride * t;
+    a.scan_up = t > 0 ? h.id.p + stride * (t - 1) : nullptr;
+    a.scan_dn = t < L ? h.id.p + stride * (t + 1) : nullptr;
+    a.count = counts[t];
+    a.count_up = t > 0 ? counts[t - 1] : 0;
+    a.count_dn = t < L ? counts[t + 1] : 0;
+    a.off = loff[t];
+    a.off_up = t > 0 ? loff[t - 1] : 0;
+    a.off_dn = t < L ? loff[t + 1] : 0;
+    a.shift = 0;
+    a.d = 0;
+    a.leaf = 0;
+    a.level = t;
+    hipLaunchKernelGGL(k_level_cells_forest, dim3(div_up(counts[t], T)), dim3(T), 0, h.stream, a, h.body4.p, key, n,
+                       nc, fsd, h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.cKey.p,
+                       h.cLevel.p);
+  }
+  h.lvl_off.assign(loff.begin(), loff.begin() + L + 2);
+  h.lvl_cells.resize(nc);
+  hipLaunchKernelGGL(k_iota, dim3(div_up(nc, T)), dim3(T), 0, h.stream, h.lvl_cells.p, nc);
+  EXA_HIP(hipGetLastError());
+}
+
 }  // namespace exafmm

@@ -24,6 +24,7 @@ EXPORTS = (
     "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
     "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
     "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald", "exafmm_hip_count_p2p_bodies",
+    "exafmm_hip_evaluate_batch",
 )
 
 
@@ -97,6 +98,7 @@ def load():
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]
+    L.exafmm_hip_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.exafmm_hip_get_dipole.argtypes = [vp, vp, vp]
     L.exafmm_hip_dipole_correction.argtypes = [vp, vp, ip, dp]
     L.exafmm_hip_ewald.argtypes = [vp, ip, dp, dp, dp, dp, ip, vp]
@@ -268,6 +270,17 @@ class FMM:
         s = Stats()
         self._check(self.lib.exafmm_hip_get_stats(self.h, ctypes.byref(s)), "get_stats")
         return s.as_dict()
+
+    def evaluate_batch(self, systems):
+        """Evaluate independent systems (a list of n_s x 4 arrays x, y, z, q)
+        in one batched pass; returns the list of n_s x 4 results."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 4) for x in systems]
+        n = np.array([a.shape[0] for a in arrs], dtype=np.int32)
+        x = np.ascontiguousarray(np.concatenate(arrs, axis=0))
+        out = np.empty_like(x)
+        self._check(self.lib.exafmm_hip_evaluate_batch(self.h, len(arrs), n.ctypes.data, x.ctypes.data,
+                                                       out.ctypes.data), "evaluate_batch")
+        return np.split(out, np.cumsum(n)[:-1])
 
     def count_p2p_bodies(self):
         """Target-source body pairs of the current P2P list."""

@@ -157,6 +157,17 @@ int exafmm_hip_dipole_correction(exafmm_hip_handle *h, const double *dipole, int
 int exafmm_hip_ewald(exafmm_hip_handle *h, int ksize, double alpha, double sigma, double cutoff, double cycle,
                      int parts, double *trg);
 
+/* Batched evaluation of nsys independent systems in one pass over the GPU:
+ * system s is n[s] bodies, concatenated in xyzq (sum n[s] x (x, y, z, q));
+ * trg receives sum n[s] x (phi, dphi/dx, dphi/dy, dphi/dz) in the same order.
+ * Each system gets its own uniform tree (its own bounds and depth, as a
+ * single-system evaluation would build it); the trees form one forest whose
+ * traversal never pairs cells of different systems.  Requires builder 0,
+ * images 0, basis 0.  The reference evaluates one system per call
+ * (examples/serial.cxx:142-156); this extends that call sequence to many
+ * systems at once. */
+int exafmm_hip_evaluate_batch(exafmm_hip_handle *h, int nsys, const int *n, const double *xyzq, double *trg);
+
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 
 /* Target-source body pairs of the current P2P list (a reduction on the
