@@ -100,6 +100,172 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": a.numel() * 8, "ok": bool(b[-1].item() == 1.0)}), flush=True)
 
 
+def rel_err(res, ref):
+    pot = float(np.sqrt(np.sum((res[:, 0] - ref[:, 0]) ** 2) / np.sum(ref[:, 0] ** 2)))
+    acc = float(np.sqrt(np.sum((res[:, 1:] - ref[:, 1:]) ** 2) / np.sum(ref[:, 1:] ** 2)))
+    return pot, acc
+
+
+def bench_batched(args):
+    """256 independent systems, N log-uniform in [4096, 262144], half uniform
+    cube, half uniform on the unit sphere surface, zero-mean charges; P = 8,
+    theta = 0.5; ncrit autotuned over {32, 64, 128}.  value = total bodies /
+    device time of one batched evaluation (HIP events, inputs resident);
+    compared with one system of the same total N (uniform cube)."""
+    rng = np.random.default_rng(2024)
+    sizes = np.floor(np.exp(rng.uniform(np.log(4096), np.log(262144), 256))).astype(np.int64)
+    systems = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            x = rng.uniform(-1, 1, (n, 3))
+        else:
+            v = rng.normal(size=(n, 3))
+            x = v / np.linalg.norm(v, axis=1)[:, None]
+        q = rng.uniform(-0.5, 0.5, n)
+        q = (q - q.mean()) / n
+        systems.append(np.concatenate([x, q[:, None]], axis=1))
+    total = int(sizes.sum())
+    steps = max(1, args.steps)
+    tune = {}
+    best = None
+    for ncrit in (32, 64, 128):
+        f = exafmm_hip.FMM(P=8, theta=0.5, ncrit=ncrit)
+        out = f.evaluate_batch(systems)  # warm-up
+        dev, wall = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            out = f.evaluate_batch(systems)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            dev.append(f.stats()["t_total"])
+        ms = float(np.median(dev))
+        tune[ncrit] = {"ms_device": round(ms, 3), "ms_wall_incl_host_copies": round(float(np.median(wall)), 3)}
+        if best is None or ms < best[1]:
+            best = (ncrit, ms, out, f)
+        else:
+            f.close()
+    ncrit, ms, out, f = best
+    # Per-system error against FP64 direct summation on 64 sampled targets.
+    errs = []
+    for x, r in zip(systems, out):
+        idx = rng.choice(x.shape[0], 64, replace=False)
+        ref = f.direct_arrays(x[idx], x)
+        errs.append(rel_err(r[idx], ref))
+    errs = np.array(errs)
+    f.close()
+    # One system of the same total N.
+    g = exafmm_hip.FMM(P=8, theta=0.5, ncrit=ncrit)
+    g.set_bodies(unit_cube(total, 99))
+    g.evaluate()
+    sdev = []
+    for _ in range(steps):
+        g.evaluate()
+        sdev.append(g.stats()["t_total"])
+    g.close()
+    sms = float(np.median(sdev))
+    line = {
+        "metric": "batched FMM particles/s: 256 independent systems, N log-uniform in [4096, 262144], P=8, theta=0.5",
+        "value": total / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": steps,
+        "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (seeded; half uniform cube, half uniform on the unit sphere surface; zero-mean charges)",
+        "config": {"workload": "batched evaluation, one forest of per-system uniform trees", "systems": len(sizes),
+                   "total_N": total, "N_min": int(sizes.min()), "N_max": int(sizes.max()), "P": 8, "theta": 0.5,
+                   "ncrit": ncrit},
+        "autotune_ncrit": tune,
+        "single_system_same_N": {"value": total / (sms * 1e-3), "ms": sms, "workload": "uniform cube, same N, ncrit"},
+        "ratio_vs_single_system": (total / (ms * 1e-3)) / (total / (sms * 1e-3)),
+        "rel_l2_err_per_system": {"pot_max": float(errs[:, 0].max()), "acc_max": float(errs[:, 1].max()),
+                                  "acc_median": float(np.median(errs[:, 1])), "targets_per_system": 64},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def plummer(n, seed):
+    """Plummer sphere, scale radius 1, radius cut at 10, isotropic; charges 1/N."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((0, 3))
+    while out.shape[0] < n:
+        m = int((n - out.shape[0]) * 1.05) + 16
+        u = rng.uniform(0, 1, m)
+        r = 1.0 / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        v = rng.normal(size=(m, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        out = np.concatenate([out, (v * r[:, None])[r < 10]], axis=0)
+    x = out[:n]
+    return np.concatenate([x, np.full((n, 1), 1.0 / n)], axis=1)
+
+
+def bench_plummer(args):
+    """N = 4,194,304 Plummer bodies (adaptive tree), P in {4, 8, 12},
+    theta = 0.4, FP64 and mixed; error vs FP64 direct on 1024 targets."""
+    n = 1 << 22
+    x = plummer(n, 5)
+    rng = np.random.default_rng(9)
+    tgt = rng.choice(n, 1024, replace=False).astype(np.int32)
+    rows = {}
+    for P in (4, 8, 12):
+        for prec in ("fp64", "mixed"):
+            f = exafmm_hip.FMM(P=P, theta=0.4, ncrit=64, builder="adaptive", precision=prec)
+            f.set_bodies(x)
+            f.evaluate()
+            dev = []
+            for _ in range(max(1, args.steps)):
+                f.evaluate()
+                dev.append(f.stats()["t_total"])
+            res = f.results()[tgt]
+            ref = f.direct(tgt)
+            pot, acc = rel_err(res, ref)
+            ms = float(np.median(dev))
+            rows[f"P{P}_{prec}"] = {"ms": round(ms, 3), "particles_per_s": n / (ms * 1e-3), "rel_l2_pot": pot,
+                                   "rel_l2_acc": acc}
+            f.close()
+    print(json.dumps({"metric": "FMM particles/s, Plummer N=4M, P in {4,8,12}, FP64 and mixed",
+                      "config": {"N": n, "theta": 0.4, "ncrit": 64, "tree": "adaptive", "targets": 1024,
+                                 "data": "synthetic Plummer (seeded), charges 1/N"},
+                      "results": rows}), flush=True)
+
+
+def bench_periodic(args):
+    """N = 128^3 jittered rock-salt lattice in [0, 2 pi)^3, images = 3,
+    theta = 0.35, spherical P = 10 and Cartesian P = 6; FMM (+ dipole
+    correction) against the Ewald sum (alpha = 10/cycle, ksize = 11,
+    cutoff = cycle/2): relative error of sum q phi and of the gradient."""
+    m = 128
+    cyc = 2 * np.pi
+    h = cyc / m
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(4)
+    x = np.concatenate([(g + 0.5) * h + rng.uniform(-0.25, 0.25, g.shape) * h,
+                        np.where(g.sum(1) % 2 == 0, 1.0, -1.0)[:, None]], axis=1)
+    rows = {}
+    ewald = None
+    for basis, P in (("spherical", 10), ("cartesian", 6)):
+        f = exafmm_hip.FMM(P=P, theta=0.35, ncrit=64, images=3, cycle=cyc, basis=basis)
+        f.set_bodies(x)
+        f.evaluate()
+        dev = []
+        for _ in range(max(1, args.steps)):
+            f.evaluate()
+            dev.append(f.stats()["t_total"])
+        d = f.dipole()
+        f.dipole_correction(d, x.shape[0], cyc)
+        r = f.results()
+        if ewald is None:
+            t0 = time.perf_counter()
+            ewald = f.ewald(11, 10 / cyc, 0.25 / np.pi, cyc / 2, cyc)
+            t_ew = time.perf_counter() - t0
+        q = x[:, 3]
+        ps, pe = float(np.sum(q * r[:, 0])), float(np.sum(q * ewald[:, 0]))
+        acc = float(np.sqrt(np.sum((r[:, 1:] - ewald[:, 1:]) ** 2) / np.sum(ewald[:, 1:] ** 2)))
+        ms = float(np.median(dev))
+        rows[f"{basis}_P{P}"] = {"ms": round(ms, 3), "particles_per_s": x.shape[0] / (ms * 1e-3),
+                                 "rel_err_sum_q_phi": abs(ps - pe) / abs(pe), "rel_l2_grad": acc}
+        f.close()
+    print(json.dumps({"metric": "periodic FMM particles/s, 128^3 lattice, images=3, vs Ewald",
+                      "config": {"N": int(x.shape[0]), "theta": 0.35, "images": 3, "ewald_s": round(t_ew, 3),
+                                 "ewald": {"ksize": 11, "alpha": 10 / cyc, "cutoff": cyc / 2}},
+                      "results": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,9 +279,17 @@ def main():
     ap.add_argument("--no-mixed", dest="with_mixed", action="store_false")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--config", default="headline", choices=["headline", "batched", "plummer", "periodic"],
+                    help="headline: the N=1M P=10 line; the others measure the batched, Plummer and periodic workloads")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.config != "headline":
+        if args.gpus != 1:
+            print("bench.py measures one GPU: run with --gpus 1.", file=sys.stderr)
+            sys.exit(2)
+        {"batched": bench_batched, "plummer": bench_plummer, "periodic": bench_periodic}[args.config](args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; multi-GPU sharding is design-only "
