@@ -227,10 +227,12 @@ __global__ void k_tile_hist(int nc, const int *__restrict__ key, int *__restrict
   if (p == nc - 1 || key[p + 1] != k) atomicAdd(&hist[k], p + 1);  // one end per group
 }
 
-// gb[0..255] = group start in sorted order, gb[256..511] = first tile of the group.
+// gb[0..255] = group start in sorted order, gb[256..511] = first chunk of
+// the group; tmap[chunk] = tile number (chunks of the 8 octant groups of a
+// level interleaved, so consecutive tiles cover one region of space).
 __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *__restrict__ sval,
-                              const int *__restrict__ gb, int *__restrict__ tile_tgt, int *__restrict__ tile_of,
-                              int *__restrict__ col_of) {
+                              const int *__restrict__ gb, const int *__restrict__ tmap, int *__restrict__ tile_tgt,
+                              int *__restrict__ tile_of, int *__restrict__ col_of) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nc) return;
   const int g = skey[p], c = sval[p];
@@ -239,7 +241,7 @@ __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *_
     return;
   }
   const int r = p - gb[g];
-  const int t = gb[256 + g] + r / 64, col = r % 64;
+  const int t = tmap[gb[256 + g] + r / 64], col = r % 64;
   tile_tgt[t * 64 + col] = c;
   tile_of[c] = t;
   col_of[c] = col;
@@ -603,8 +605,20 @@ static void lattice_run(Handle &h, M2LList &irr) {
     start += g < 255 ? ghist[g] : 0;
     ntiles += g < 255 ? div_up(ghist[g], 64) : 0;
   }
-  h.lat_gb.resize(512);
+  // Tile numbers: per level, chunk q of octant 0..7, then chunk q + 1, ...
+  std::vector<int> tmap(ntiles > 0 ? ntiles : 1);
+  for (int l = 0, id = 0; l < 32; l++) {
+    int qmax = 0;
+    for (int o = 0; o < 8 && l * 8 + o < 255; o++) qmax = std::max(qmax, div_up(ghist[l * 8 + o], 64));
+    for (int q = 0; q < qmax; q++)
+      for (int o = 0; o < 8 && l * 8 + o < 255; o++) {
+        const int g = l * 8 + o;
+        if (q < div_up(ghist[g], 64)) tmap[gb[256 + g] + q] = id++;
+      }
+  }
+  h.lat_gb.resize(512 + tmap.size());
   EXA_HIP(hipMemcpyAsync(h.lat_gb.p, gb, sizeof(gb), hipMemcpyHostToDevice, h.stream));
+  EXA_HIP(hipMemcpyAsync(h.lat_gb.p + 512, tmap.data(), tmap.size() * sizeof(int), hipMemcpyHostToDevice, h.stream));
   h.lat_tile_tgt.resize((size_t)(ntiles > 0 ? ntiles : 1) * 64);
   h.lat_tile_of.resize(nc);
   h.lat_col_of.resize(nc);
@@ -612,7 +626,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
     hipLaunchKernelGGL(k_fill_int, dim3(div_up((int64_t)ntiles * 64, TPB)), dim3(TPB), 0, h.stream, h.lat_tile_tgt.p,
                        (int64_t)ntiles * 64, -1);
   hipLaunchKernelGGL(k_tile_assign, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, sval, h.lat_gb.p,
-                     h.lat_tile_tgt.p, h.lat_tile_of.p, h.lat_col_of.p);
+                     h.lat_gb.p + 512, h.lat_tile_tgt.p, h.lat_tile_of.p, h.lat_col_of.p);
   // 4. Steps: (tile, offset) with at least one pair, compacted per tile.
   const int64_t nsteps_all = (int64_t)ntiles * noff;
   h.lat_act.resize(nsteps_all + 1);
