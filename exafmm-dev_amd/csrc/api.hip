@@ -167,9 +167,12 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   // EXAFMM_HIP_M2L=pairs keeps every M2L pair on the per-pair kernel.
   if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
+  if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_waves = std::atoi(m);
+  if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
   H->h.nterm = cfg->basis == 1 ? cfg->P * (cfg->P + 1) * (cfg->P + 2) / 6 : cfg->P * (cfg->P + 1) / 2;
   try {
     set_device(H->h);
+    EXA_HIP(hipDeviceGetAttribute(&H->h.ncu, hipDeviceAttributeMultiprocessorCount, H->h.cfg.device));
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream2, hipStreamNonBlocking));
     for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));

@@ -109,6 +109,7 @@ struct Handle {
   DBuf<int> p2p_off, p2p_src, m2l_off, m2l_src;
   DBuf<unsigned char> p2p_img, m2l_img;
   DBuf<int> p2p_tgt, m2l_tgt;      // compact target lists
+  DBuf<int> p2p_next;              // P2P work counter (next target leaf)
   int np2p_tgt = 0, nm2l_tgt = 0;
   DBuf<int> raw_mi, raw_mj, raw_pi, raw_pj;  // emission-order lists
   DBuf<unsigned char> raw_mimg, raw_pimg;
@@ -131,6 +132,13 @@ struct Handle {
   bool lattice_m2l = true;
   // M2M / L2L as batched MFMA products with per-octant operators.
   bool updown_mfma = true, ud_ready = false;
+  // Co-residency of P2P (VALU) and the lattice M2L (matrix cores): P2P as a
+  // grid of p2p_waves waves per SIMD that claims target leaves from a
+  // counter (0 = one wave per leaf; -1 = 2 when the lattice M2L runs beside
+  // it, which leaves each CU room for one lattice workgroup, else 0), the
+  // lattice kernel as lat_wg_cu workgroups per CU looping over (tile,
+  // segment) items (0 = one workgroup per item).  ncu = compute units.
+  int p2p_waves = -1, lat_wg_cu = 0, ncu = 0;
   DBuf<double> ud_ops;
   DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
   DBuf<double> lat_part;
@@ -192,6 +200,7 @@ void run_ropt(Handle &h);
 bool run_m2l_lattice(Handle &h, M2LList &irr);
 void run_lattice_mscale(Handle &h, hipStream_t st);
 bool updown_mfma_enabled(const Handle &h);
+bool lattice_applies(const Handle &h);
 void run_m2m_level_mfma(Handle &h, const int *cells, int cnt, hipStream_t st);
 void run_l2l_level_mfma(Handle &h, const int *cells, int cnt);
 void run_upward_cart(Handle &h);

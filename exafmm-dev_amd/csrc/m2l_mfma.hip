@@ -406,8 +406,9 @@ typedef __attribute__((address_space(1))) void glb_void;
 // go to part[] for k_lat_reduce.
 template <int P, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 10 ? 2 : 1, 2)))
-    k_m2l_mfma(int nc, const int *__restrict__ step0, const int *__restrict__ alist, const int *__restrict__ slot,
-               const R *__restrict__ T, const R *__restrict__ Ms, double *__restrict__ part) {
+    k_m2l_mfma(int nc, int nitems, const int *__restrict__ step0, const int *__restrict__ alist,
+               const int *__restrict__ slot, const R *__restrict__ T, const R *__restrict__ Ms,
+               double *__restrict__ part) {
   using S = MfmaShape<P>;
   using A = MfmaAcc<R>;
   using TP = TPad<P, R>;
@@ -416,87 +417,92 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
   constexpr int TEP = TP::TEP, CH = TP::CH, NCH = TP::NCH, VL = 16 / (int)sizeof(R);
   __shared__ __attribute__((aligned(16))) R sT0[TEP];
   __shared__ __attribute__((aligned(16))) R sT1[TEP];
-  const int tile = blockIdx.x >> 3, seg = blockIdx.x & 7;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int col = wave * 16 + (lane & 15);
-  const int kq = lane >> 4;
-  const int a0 = step0[tile], a1 = step0[tile + 1];
-  const int i0 = a0 + (a1 - a0) * seg / kLatSeg, i1 = a0 + (a1 - a0) * (seg + 1) / kLatSeg;
-  typename A::type acc[RB];
-  v4d acc64[F32 ? RB : 1];
-#pragma unroll
-  for (int rb = 0; rb < RB; rb++) acc[rb] = typename A::type{0, 0, 0, 0};
-#pragma unroll
-  for (int rb = 0; rb < (F32 ? RB : 1); rb++) acc64[rb] = v4d{0, 0, 0, 0};
-  // T(d) of step i into an LDS buffer: chunk ch by wave ch % 4.
-  auto fetch_T = [&](R *dst, int i) {
-    const R *Tn = T + (size_t)alist[i] * TEP + lane * VL;
-    for (int ch = wave; ch < NCH; ch += 4)
-      __builtin_amdgcn_global_load_lds((glb_void *)(Tn + ch * CH), (lds_void *)(dst + ch * CH), 16, 0, 0);
-  };
-  R b[KC];
-  int sn = -1;
-  // Step i from buffer CUR = (i - i0) & 1 while step i + 1 is fetched into
-  // the other one.
-  auto step = [&](auto curc, int i) {
-    constexpr int CUR = decltype(curc)::value;
-    const R *sTc = CUR ? sT1 : sT0;
-    R *sTn = CUR ? sT0 : sT1;
-    const bool more = i + 1 < i1;
-    R bn[KC];
-    int snn = -1;
-    if (more) {
+  // Items (tile, segment) = (item / 8, item % 8); a grid that is a multiple
+  // of 8 keeps every item of a segment on one XCD.
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int tile = item >> 3, seg = item & 7;
+    const int col = wave * 16 + (lane & 15);
+    const int kq = lane >> 4;
+    const int a0 = step0[tile], a1 = step0[tile + 1];
+    const int i0 = a0 + (a1 - a0) * seg / kLatSeg, i1 = a0 + (a1 - a0) * (seg + 1) / kLatSeg;
+    typename A::type acc[RB];
+    v4d acc64[F32 ? RB : 1];
+  #pragma unroll
+    for (int rb = 0; rb < RB; rb++) acc[rb] = typename A::type{0, 0, 0, 0};
+  #pragma unroll
+    for (int rb = 0; rb < (F32 ? RB : 1); rb++) acc64[rb] = v4d{0, 0, 0, 0};
+    // T(d) of step i into an LDS buffer: chunk ch by wave ch % 4.
+    auto fetch_T = [&](R *dst, int i) {
+      const R *Tn = T + (size_t)alist[i] * TEP + lane * VL;
+      for (int ch = wave; ch < NCH; ch += 4)
+        __builtin_amdgcn_global_load_lds((glb_void *)(Tn + ch * CH), (lds_void *)(dst + ch * CH), 16, 0, 0);
+    };
+    R b[KC];
+    int sn = -1;
+    // Step i from buffer CUR = (i - i0) & 1 while step i + 1 is fetched into
+    // the other one.
+    auto step = [&](auto curc, int i) {
+      constexpr int CUR = decltype(curc)::value;
+      const R *sTc = CUR ? sT1 : sT0;
+      R *sTn = CUR ? sT0 : sT1;
+      const bool more = i + 1 < i1;
+      R bn[KC];
+      int snn = -1;
+      if (more) {
+        const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
+  #pragma unroll
+        for (int kc = 0; kc < KC; kc++) bn[kc] = src[kc * 4];
+        fetch_T(sTn, i + 1);
+        if (i + 2 < i1) snn = slot[(size_t)(i + 2) * 64 + col];
+      }
+  #pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+  #pragma unroll
+        for (int kc = 0; kc < S::kc_end(rb); kc++)
+          acc[rb] = A::mac(sTc[(S::base(rb) + kc) * 64 + lane], b[kc], acc[rb]);
+      }
+      if (F32) {
+  #pragma unroll
+        for (int rb = 0; rb < (F32 ? RB : 1); rb++)
+          for (int r = 0; r < 4; r++) {
+            acc64[rb][r] += (double)acc[rb][r];
+            acc[rb][r] = 0;
+          }
+      }
+      if (more) {
+  #pragma unroll
+        for (int kc = 0; kc < KC; kc++) b[kc] = bn[kc];
+        sn = snn;
+      }
+      __builtin_amdgcn_s_waitcnt(kVmcnt0);  // this wave's DMA and loads have landed
+      __syncthreads();
+    };
+    // Prologue: step i0's T and columns.
+    if (i0 < i1) {
+      fetch_T(sT0, i0);
+      sn = slot[(size_t)i0 * 64 + col];
       const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
-#pragma unroll
-      for (int kc = 0; kc < KC; kc++) bn[kc] = src[kc * 4];
-      fetch_T(sTn, i + 1);
-      if (i + 2 < i1) snn = slot[(size_t)(i + 2) * 64 + col];
+  #pragma unroll
+      for (int kc = 0; kc < KC; kc++) b[kc] = src[kc * 4];
+      sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
     }
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
-#pragma unroll
-      for (int kc = 0; kc < S::kc_end(rb); kc++) acc[rb] = A::mac(sTc[(S::base(rb) + kc) * 64 + lane], b[kc], acc[rb]);
-    }
-    if (F32) {
-#pragma unroll
-      for (int rb = 0; rb < (F32 ? RB : 1); rb++)
-        for (int r = 0; r < 4; r++) {
-          acc64[rb][r] += (double)acc[rb][r];
-          acc[rb][r] = 0;
-        }
-    }
-    if (more) {
-#pragma unroll
-      for (int kc = 0; kc < KC; kc++) b[kc] = bn[kc];
-      sn = snn;
-    }
-    __builtin_amdgcn_s_waitcnt(kVmcnt0);  // this wave's DMA and loads have landed
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
     __syncthreads();
-  };
-  // Prologue: step i0's T and columns.
-  if (i0 < i1) {
-    fetch_T(sT0, i0);
-    sn = slot[(size_t)i0 * 64 + col];
-    const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) b[kc] = src[kc * 4];
-    sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
-  }
-  __builtin_amdgcn_s_waitcnt(kVmcnt0);
-  __syncthreads();
-  for (int i = i0; i < i1; i += 2) {
-    step(std::integral_constant<int, 0>{}, i);
-    if (i + 1 < i1) step(std::integral_constant<int, 1>{}, i + 1);
-  }
-  double *pp = part + (size_t)blockIdx.x * (RB * 16) * 64;
-#pragma unroll
-  for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int row = rb * 16 + d_row<R>(lane, r);
-      const double v = F32 ? acc64[F32 ? rb : 0][r] : (double)acc[rb][r];
-      pp[(size_t)row * 64 + col] = v;
+    for (int i = i0; i < i1; i += 2) {
+      step(std::integral_constant<int, 0>{}, i);
+      if (i + 1 < i1) step(std::integral_constant<int, 1>{}, i + 1);
     }
+    double *pp = part + (size_t)item * (RB * 16) * 64;
+  #pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+  #pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rb * 16 + d_row<R>(lane, r);
+        const double v = F32 ? acc64[F32 ? rb : 0][r] : (double)acc[rb][r];
+        pp[(size_t)row * 64 + col] = v;
+      }
+  }
 }
 
 // L of each tile column += u^-(j+1) * (sum of the segments' partials, in
@@ -698,22 +704,27 @@ static void lattice_run(Handle &h, M2LList &irr) {
   // 7. The contraction (timed by ev[12..13]) and the reduction.
   if (noff == 0 || ntiles == 0 || nact == 0) return;
   h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
-  const dim3 grid(ntiles * kLatSeg);
+  // One workgroup per item, or lat_wg_cu workgroups per CU looping over the
+  // items (room left on every CU for the P2P waves of the other stream).
+  const int nitems = ntiles * kLatSeg;
+  int ngrid = nitems;
+  if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, h.lat_wg_cu * h.ncu);
+  const dim3 grid(ngrid);
   EXA_HIP(hipEventRecord(h.ev[12], h.stream));
   h.lat_timed = true;
   if (f32)
-    hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
-                       h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
+                       h.lat_alist.p, h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
   else
-    hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, h.lat_step0.p, h.lat_alist.p,
-                       h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
+    hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
+                       h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
   EXA_HIP(hipEventRecord(h.ev[13], h.stream));
   hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
                      h.lat_part.p, h.L.p);
   EXA_HIP(hipGetLastError());
 }
 
-static bool lattice_applies(const Handle &h) {
+bool lattice_applies(const Handle &h) {
   return h.cfg.basis == 0 && !h.cfg.mass && h.lattice_m2l && h.cfg.P >= 4 && h.cfg.P <= 12 && (h.cfg.P % 2) == 0;
 }
 

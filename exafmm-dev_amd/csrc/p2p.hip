@@ -1,4 +1,7 @@
-// P2P over the CSR interaction list: one 64-lane workgroup per target leaf.
+// P2P over the CSR interaction list: a 64-lane workgroup works on one target
+// leaf at a time, taking the next leaf from a work counter when done (the
+// grid is one wave per leaf, or smaller so that the lattice M2L kernel of the
+// other stream fits beside it on every CU).
 //
 // Reference: kernels/LaplaceP2PCPU.cxx:141-169 (pair) and :172-331 (self).
 // Per pair: dX = x_i - x_j - X_periodic, R2 = |dX|^2, skipped when R2 == 0;
@@ -16,6 +19,9 @@
 // Source bodies are staged through LDS (consecutive list entries, each
 // shifted by its periodic image); the G partial sums of a target are
 // combined in a fixed order at the end (deterministic, no atomics).
+#include <algorithm>
+#include <type_traits>
+
 #include "internal.h"
 
 namespace exafmm {
@@ -27,6 +33,8 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
 }
 
 constexpr int kP2PStage = 256;  // source bodies staged in LDS per fill
+static_assert(kP2PStage % 32 == 0, "stages are padded to multiples of 4 G <= 32");
+constexpr double kFar = 1e100;  // position of the zero-charge padding bodies
 
 // 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
 // higher-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, which
@@ -116,6 +124,16 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
   return cnt;
 }
 
+// The next target leaf of a one-wave workgroup: counter + grid size (the
+// counter starts at 0 and the grid's own first leaves are 0 .. grid - 1).
+__device__ inline int next_leaf(int *next, int lane) {
+  int v = 0;
+  if (lane == 0) v = atomicAdd(next, 1);
+  return __shfl(v, 0) + (int)gridDim.x;
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
 // The two target passes of a block of n <= 64 targets (see above).
 struct TargetPasses {
   int TA, TB, r;  // r = targets of pass B (0: no pass B)
@@ -129,80 +147,107 @@ struct TargetPasses {
 };
 
 __global__ void __launch_bounds__(64)
-    k_p2p_f64(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
-              const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
+    k_p2p_f64(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
+              const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
               const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
   __shared__ double4 sb[kP2PStage];
   __shared__ double4 sacc[64];
   __shared__ SrcDesc sd;
-  const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
-  const int tb0 = cBody[c], tn = cNBody[c];
-  const int e0 = off[c], e1 = off[c + 1];
-  auto pair = [](const double4 &xi, const double4 &xj, double4 &acc) {
-    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double rs = rsqrt_pos(r2);  // unconditional: a select, not a branch, skips R2 == 0
-    const double invr = r2 > 0 ? rs : 0.0;
-    const double qinvr = xj.w * invr;
-    const double qinvr3 = qinvr * invr * invr;
-    acc.x += qinvr;
-    acc.y += dx * qinvr3;
-    acc.z += dy * qinvr3;
-    acc.w += dz * qinvr3;
-  };
-  // Sum the G group partials of the pass's targets in group order.
-  auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
-    __syncthreads();
-    sacc[lane] = acc;
-    __syncthreads();
-    if (lane < n) {
-      double4 s = sacc[lane];
-      for (int gg = 1; gg < 64 / T; gg++) {
-        const double4 o = sacc[gg * T + lane];
-        s.x += o.x;
-        s.y += o.y;
-        s.z += o.z;
-        s.w += o.w;
+  // Target leaves: blockIdx.x first, then the next unclaimed one of the
+  // work counter (leaves differ in cost; the grid may be smaller than ntl).
+  for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
+    const int c = tgt_list[bi];
+    const int tb0 = cBody[c], tn = cNBody[c];
+    const int e0 = off[c], e1 = off[c + 1];
+    // MASK: the stage may hold a body at the target's own position (its own
+    // leaf, or a periodic image), so R2 == 0 is skipped by a select; other
+    // stages have R2 > 0 for every pair (equal positions share a Morton key
+    // and hence a leaf).
+    auto pair = [](auto mask, const double4 &xi, const double4 &xj, double4 &acc) {
+      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double rs = rsqrt_pos(r2);
+      const double invr = decltype(mask)::value ? (r2 > 0 ? rs : 0.0) : rs;
+      const double qinvr = xj.w * invr;
+      const double qinvr3 = qinvr * invr * invr;
+      acc.x += qinvr;
+      acc.y += dx * qinvr3;
+      acc.z += dy * qinvr3;
+      acc.w += dz * qinvr3;
+    };
+    // Sum the G group partials of the pass's targets in group order.
+    auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
+      __syncthreads();
+      sacc[lane] = acc;
+      __syncthreads();
+      if (lane < n) {
+        double4 s = sacc[lane];
+        for (int gg = 1; gg < 64 / T; gg++) {
+          const double4 o = sacc[gg * T + lane];
+          s.x += o.x;
+          s.y += o.y;
+          s.z += o.z;
+          s.w += o.w;
+        }
+        double4 tv = trg[tb0 + tfirst + lane];
+        tv.x += s.x;
+        tv.y -= s.y;
+        tv.z -= s.z;
+        tv.w -= s.w;
+        trg[tb0 + tfirst + lane] = tv;
       }
-      double4 tv = trg[tb0 + tfirst + lane];
-      tv.x += s.x;
-      tv.y -= s.y;
-      tv.z -= s.z;
-      tv.w -= s.w;
-      trg[tb0 + tfirst + lane] = tv;
-    }
-  };
-  for (int tbase = 0; tbase < tn; tbase += 64) {
-    const int nt = min(64, tn - tbase);
-    const TargetPasses tp(nt);
-    const int GA = 64 / tp.TA, GB = 64 / tp.TB;
-    const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
-    const double4 xa = ia < nt ? body[tb0 + tbase + ia] : make_double4(0, 0, 0, 0);
-    const double4 xb = ib < tp.r ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
-    double4 acca = make_double4(0, 0, 0, 0), accb = make_double4(0, 0, 0, 0);
-    SrcWalker w{e0, 0, e0, 0};
-    while (w.e < e1) {
-      // Fill: consecutive source leaves (or parts of one) up to kP2PStage
-      // bodies, each shifted by its periodic image.
-      const int cnt = fill_stage<kP2PStage>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
-        double px, py, pz;
-        image_shift(im, cycle, px, py, pz);
-        double4 v = body[b];
-        v.x += px;
-        v.y += py;
-        v.z += pz;
-        sb[k] = v;
-      });
-#pragma unroll 4
-      for (int k = ga; k < cnt; k += GA) pair(xa, sb[k], acca);
-      if (tp.r > 0) {
-#pragma unroll 4
-        for (int k = gb; k < cnt; k += GB) pair(xb, sb[k], accb);
+    };
+    for (int tbase = 0; tbase < tn; tbase += 64) {
+      const int nt = min(64, tn - tbase);
+      const TargetPasses tp(nt);
+      const int GA = 64 / tp.TA, GB = 64 / tp.TB;
+      const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
+      const double4 xa = ia < nt ? body[tb0 + tbase + ia] : make_double4(0, 0, 0, 0);
+      const double4 xb = ib < tp.r ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
+      double4 acca = make_double4(0, 0, 0, 0), accb = make_double4(0, 0, 0, 0);
+      SrcWalker w{e0, 0, e0, 0};
+      while (w.e < e1) {
+        // Fill: consecutive source leaves (or parts of one) up to kP2PStage
+        // bodies, each shifted by its periodic image.
+        bool own = false;
+        const int cnt = fill_stage<kP2PStage>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
+          double px, py, pz;
+          image_shift(im, cycle, px, py, pz);
+          double4 v = body[b];
+          v.x += px;
+          v.y += py;
+          v.z += pz;
+          sb[k] = v;
+          own |= (b >= tb0 && b < tb0 + tn) || im != 13;
+        });
+        // Pad the stage to a multiple of 4 G (G of either pass, at most 8; the
+        // stage size is a multiple of 32) with zero charges far away, which
+        // add exactly zero, so both passes run whole groups of 4 sources.
+        const int g4 = 4 * (tp.r > 0 ? GB : GA);
+        const int cntp = (cnt + g4 - 1) / g4 * g4;
+        for (int k = cnt + lane; k < cntp; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
+        __syncthreads();
+        auto run = [&](auto mask) {
+          for (int j = 0; j < cntp; j += 4 * GA) {
+  #pragma unroll
+            for (int u = 0; u < 4; u++) pair(mask, xa, sb[j + ga + u * GA], acca);
+          }
+          if (tp.r > 0) {
+            for (int j = 0; j < cntp; j += 4 * GB) {
+  #pragma unroll
+              for (int u = 0; u < 4; u++) pair(mask, xb, sb[j + gb + u * GB], accb);
+            }
+          }
+        };
+        if (__any(own))
+          run(std::true_type{});
+        else
+          run(std::false_type{});
       }
+      combine(acca, tp.TA, min(nt, tp.TA), tbase);
+      if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
     }
-    combine(acca, tp.TA, min(nt, tp.TA), tbase);
-    if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
   }
 }
 
@@ -220,6 +265,7 @@ __global__ void __launch_bounds__(64)
 constexpr int kP2PStageF = 256;
 static_assert(kP2PStageF % 64 == 0, "stage tiles are padded to multiples of 64");
 constexpr int kSeg = 8;  // FP32 pair terms summed before each compensated add
+constexpr float kFarF = 1e18f;  // padding position: R2 = 3e36 stays finite in FP32
 
 struct Kahan32 {
   float s = 0, c = 0;
@@ -233,117 +279,133 @@ struct Kahan32 {
 };
 
 __global__ void __launch_bounds__(64)
-    k_p2p_mixed(const int *__restrict__ tgt_list, const int *__restrict__ off, const int *__restrict__ src,
-                const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
+    k_p2p_mixed(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
+                const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
                 const double4 *__restrict__ cX, const double4 *__restrict__ body, double4 *__restrict__ trg,
                 double cycle) {
   __shared__ float4 sb[kP2PStageF];   // hi parts and charge
   __shared__ float4 sbl[kP2PStageF];  // lo parts
   __shared__ double4 sacc[64];
   __shared__ SrcDesc sd;
-  const int c = tgt_list[blockIdx.x];
   const int lane = threadIdx.x;
-  const int tb0 = cBody[c], tn = cNBody[c];
-  const double4 Xc = cX[c];
-  const int e0 = off[c], e1 = off[c + 1];
-  // A target's position relative to the leaf centre, FP32 hi + lo.
-  struct Tgt {
-    float x, y, z, xl, yl, zl;
-  };
-  auto load_tgt = [&](bool active, int b) {
-    const double4 xd = active ? body[b] : Xc;
-    const double rx = xd.x - Xc.x, ry = xd.y - Xc.y, rz = xd.z - Xc.z;
-    Tgt t;
-    t.x = (float)rx;
-    t.y = (float)ry;
-    t.z = (float)rz;
-    t.xl = (float)(rx - t.x);
-    t.yl = (float)(ry - t.y);
-    t.zl = (float)(rz - t.z);
-    return t;
-  };
-  struct Acc {
-    Kahan32 p, x, y, z;
-  };
-  // kSeg sources k0, k0 + G, ... in FP32, then one compensated add each.
-  auto segment = [&](const Tgt &t, int k0, int G, Acc &a) {
-    float pot = 0, ax = 0, ay = 0, az = 0;
-#pragma unroll
-    for (int u = 0; u < kSeg; u++) {
-      const int k = k0 + u * G;
-      const float4 xj = sb[k];
-      const float4 xl = sbl[k];
-      const float dx = (t.x - xj.x) + (t.xl - xl.x), dy = (t.y - xj.y) + (t.yl - xl.y),
-                  dz = (t.z - xj.z) + (t.zl - xl.z);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float rs = __builtin_amdgcn_rsqf(r2);
-      const float invr = r2 > 0.0f ? rs : 0.0f;
-      const float qinvr = xj.w * invr;
-      const float qinvr3 = qinvr * invr * invr;
-      pot += qinvr;
-      ax += dx * qinvr3;
-      ay += dy * qinvr3;
-      az += dz * qinvr3;
-    }
-    a.p.add(pot);
-    a.x.add(ax);
-    a.y.add(ay);
-    a.z.add(az);
-  };
-  auto combine = [&](const Acc &a, int T, int n, int tfirst) {
-    __syncthreads();
-    sacc[lane] = make_double4(a.p.value(), a.x.value(), a.y.value(), a.z.value());
-    __syncthreads();
-    if (lane < n) {
-      double4 s = sacc[lane];
-      for (int gg = 1; gg < 64 / T; gg++) {
-        const double4 o = sacc[gg * T + lane];
-        s.x += o.x;
-        s.y += o.y;
-        s.z += o.z;
-        s.w += o.w;
+  // Target leaves: blockIdx.x first, then the next unclaimed one of the
+  // work counter (leaves differ in cost; the grid may be smaller than ntl).
+  for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
+    const int c = tgt_list[bi];
+    const int tb0 = cBody[c], tn = cNBody[c];
+    const double4 Xc = cX[c];
+    const int e0 = off[c], e1 = off[c + 1];
+    // A target's position relative to the leaf centre, FP32 hi + lo.
+    struct Tgt {
+      float x, y, z, xl, yl, zl;
+    };
+    auto load_tgt = [&](bool active, int b) {
+      const double4 xd = active ? body[b] : Xc;
+      const double rx = xd.x - Xc.x, ry = xd.y - Xc.y, rz = xd.z - Xc.z;
+      Tgt t;
+      t.x = (float)rx;
+      t.y = (float)ry;
+      t.z = (float)rz;
+      t.xl = (float)(rx - t.x);
+      t.yl = (float)(ry - t.y);
+      t.zl = (float)(rz - t.z);
+      return t;
+    };
+    struct Acc {
+      Kahan32 p, x, y, z;
+    };
+    // kSeg sources k0, k0 + G, ... in FP32, then one compensated add each.
+    // MASK as in k_p2p_f64: only stages that may hold the target's own
+    // position skip R2 == 0 by a select.
+    auto segment = [&](auto mask, const Tgt &t, int k0, int G, Acc &a) {
+      float pot = 0, ax = 0, ay = 0, az = 0;
+  #pragma unroll
+      for (int u = 0; u < kSeg; u++) {
+        const int k = k0 + u * G;
+        const float4 xj = sb[k];
+        const float4 xl = sbl[k];
+        const float dx = (t.x - xj.x) + (t.xl - xl.x), dy = (t.y - xj.y) + (t.yl - xl.y),
+                    dz = (t.z - xj.z) + (t.zl - xl.z);
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float rs = __builtin_amdgcn_rsqf(r2);
+        const float invr = decltype(mask)::value ? (r2 > 0.0f ? rs : 0.0f) : rs;
+        const float qinvr = xj.w * invr;
+        const float qinvr3 = qinvr * invr * invr;
+        pot += qinvr;
+        ax += dx * qinvr3;
+        ay += dy * qinvr3;
+        az += dz * qinvr3;
       }
-      double4 tv = trg[tb0 + tfirst + lane];
-      tv.x += s.x;
-      tv.y -= s.y;
-      tv.z -= s.z;
-      tv.w -= s.w;
-      trg[tb0 + tfirst + lane] = tv;
-    }
-  };
-  for (int tbase = 0; tbase < tn; tbase += 64) {
-    const int nt = min(64, tn - tbase);
-    const TargetPasses tp(nt);
-    const int GA = 64 / tp.TA, GB = 64 / tp.TB;
-    const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
-    const Tgt ta = load_tgt(ia < nt, tb0 + tbase + ia);
-    const Tgt tb = load_tgt(ib < tp.r, tb0 + tbase + tp.TA + ib);
-    Acc acca, accb;
-    SrcWalker w{e0, 0, e0, 0};
-    while (w.e < e1) {
-      const int cnt = fill_stage<kP2PStageF>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
-        double px, py, pz;
-        image_shift(im, cycle, px, py, pz);
-        const double4 v = body[b];
-        const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
-        const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-        sb[k] = make_float4(hx, hy, hz, (float)v.w);
-        sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
-      });
-      // Pad the tile to a multiple of 64 (= kSeg G for G = 8, and a
-      // multiple of it for G < 8) with zero charges, which add exactly 0.
-      const int cntp = (cnt + 63) / 64 * 64;
-      for (int k = cnt + lane; k < cntp; k += 64) {
-        sb[k] = make_float4(0, 0, 0, 0);
-        sbl[k] = make_float4(0, 0, 0, 0);
-      }
+      a.p.add(pot);
+      a.x.add(ax);
+      a.y.add(ay);
+      a.z.add(az);
+    };
+    auto combine = [&](const Acc &a, int T, int n, int tfirst) {
       __syncthreads();
-      for (int k0 = ga; k0 < cntp; k0 += kSeg * GA) segment(ta, k0, GA, acca);
-      if (tp.r > 0)
-        for (int k0 = gb; k0 < cntp; k0 += kSeg * GB) segment(tb, k0, GB, accb);
+      sacc[lane] = make_double4(a.p.value(), a.x.value(), a.y.value(), a.z.value());
+      __syncthreads();
+      if (lane < n) {
+        double4 s = sacc[lane];
+        for (int gg = 1; gg < 64 / T; gg++) {
+          const double4 o = sacc[gg * T + lane];
+          s.x += o.x;
+          s.y += o.y;
+          s.z += o.z;
+          s.w += o.w;
+        }
+        double4 tv = trg[tb0 + tfirst + lane];
+        tv.x += s.x;
+        tv.y -= s.y;
+        tv.z -= s.z;
+        tv.w -= s.w;
+        trg[tb0 + tfirst + lane] = tv;
+      }
+    };
+    for (int tbase = 0; tbase < tn; tbase += 64) {
+      const int nt = min(64, tn - tbase);
+      const TargetPasses tp(nt);
+      const int GA = 64 / tp.TA, GB = 64 / tp.TB;
+      const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
+      const Tgt ta = load_tgt(ia < nt, tb0 + tbase + ia);
+      const Tgt tb = load_tgt(ib < tp.r, tb0 + tbase + tp.TA + ib);
+      Acc acca, accb;
+      SrcWalker w{e0, 0, e0, 0};
+      while (w.e < e1) {
+        bool own = false;
+        const int cnt = fill_stage<kP2PStageF>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
+          double px, py, pz;
+          image_shift(im, cycle, px, py, pz);
+          const double4 v = body[b];
+          const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
+          const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+          sb[k] = make_float4(hx, hy, hz, (float)v.w);
+          sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
+          own |= (b >= tb0 && b < tb0 + tn) || im != 13;
+        });
+        // Pad the tile to a multiple of kSeg G (G of either pass, at most 8;
+        // the stage size is a multiple of 64) with zero charges far away,
+        // which add exactly 0.
+        const int g8 = kSeg * (tp.r > 0 ? GB : GA);
+        const int cntp = (cnt + g8 - 1) / g8 * g8;
+        for (int k = cnt + lane; k < cntp; k += 64) {
+          sb[k] = make_float4(kFarF, kFarF, kFarF, 0);
+          sbl[k] = make_float4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        auto run = [&](auto mask) {
+          for (int j = 0; j < cntp; j += kSeg * GA) segment(mask, ta, j + ga, GA, acca);
+          if (tp.r > 0)
+            for (int j = 0; j < cntp; j += kSeg * GB) segment(mask, tb, j + gb, GB, accb);
+        };
+        if (__any(own))
+          run(std::true_type{});
+        else
+          run(std::false_type{});
+      }
+      combine(acca, tp.TA, min(nt, tp.TA), tbase);
+      if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
     }
-    combine(acca, tp.TA, min(nt, tp.TA), tbase);
-    if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
   }
 }
 
@@ -362,12 +424,23 @@ __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, co
 void run_p2p(Handle &h, hipStream_t st) {
   EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), st));
   if (h.np2p_tgt == 0) return;
+  // One wave per target leaf, or a grid of `waves` waves per SIMD (4 SIMDs
+  // per CU) that claims target leaves from a counter and leaves room on
+  // every CU: 2 P2P waves per SIMD (LDS 8 x 10.6 KB, at most 120 VGPRs each)
+  // beside one FP64 or FP32 lattice M2L workgroup (63.5 / 32 KB, 232 / 160
+  // VGPRs per wave) fit a CU's 160 KB of LDS and 512 VGPRs per SIMD lane.
+  int waves = h.p2p_waves;
+  if (waves < 0) waves = lattice_applies(h) && h.nm2l > 0 && h.cfg.P <= 10 ? 2 : 0;
+  int grid = h.np2p_tgt;
+  if (waves > 0) grid = std::min(grid, waves * 4 * h.ncu);
+  h.p2p_next.reserve(1);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, h.p2p_next.p);
   if (h.cfg.precision == 1)
-    hipLaunchKernelGGL(k_p2p_mixed, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
-                       h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+    hipLaunchKernelGGL(k_p2p_mixed, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, h.p2p_next.p, h.p2p_off.p,
+                       h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   else
-    hipLaunchKernelGGL(k_p2p_f64, dim3(h.np2p_tgt), dim3(64), 0, st, h.p2p_tgt.p, h.p2p_off.p, h.p2p_src.p,
-                       h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+    hipLaunchKernelGGL(k_p2p_f64, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, h.p2p_next.p, h.p2p_off.p,
+                       h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   EXA_HIP(hipGetLastError());
 }
 
