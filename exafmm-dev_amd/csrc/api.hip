@@ -169,6 +169,9 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_waves = std::atoi(m);
   if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
+  // Rows of the lattice T(d) table kept across evaluations (16384 rows are
+  // 0.5 GB in FP64 at P = 10).
+  if (const char *m = std::getenv("EXAFMM_HIP_TCACHE_ROWS")) H->h.lat_tc_max = std::max(1, std::atoi(m));
   H->h.nterm = cfg->basis == 1 ? cfg->P * (cfg->P + 1) * (cfg->P + 2) / 6 : cfg->P * (cfg->P + 1) / 2;
   try {
     set_device(H->h);

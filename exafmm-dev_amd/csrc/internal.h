@@ -146,6 +146,11 @@ struct Handle {
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
   DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
+  // T(d) table kept across evaluations: offset key -> row of lat_T / lat_Tf
+  // (-1: not computed yet), for expansion order lat_tc_P and FP32 or FP64.
+  DBuf<int> lat_tc_map, lat_tc_flag, lat_tc_scan, lat_tc_keys;
+  int lat_tc_rows = 0, lat_tc_P = 0, lat_tc_max = 16384;  // rows; a fuller table starts afresh
+  bool lat_tc_f32 = false;
   bool lat_ms_ready = false;     // lat_Ms / lat_Msf already queued for this evaluation
   DBuf<double> lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;

@@ -102,31 +102,24 @@ struct MfmaAcc<float> {
 };
 
 // ---------------------------------------------------------------------------
-// Pair classification: key of the lattice offset, or -1.
-__global__ void k_lat_classify(const int *__restrict__ off, int nc, const int *__restrict__ src,
+// Pair classification: key of the lattice offset, or -1 (ptgt: the target
+// cell of every CSR entry).
+__global__ void k_lat_classify(const int *__restrict__ ptgt, const int *__restrict__ src,
                                const unsigned char *__restrict__ img, int64_t np, const double4 *__restrict__ cX,
-                               const double *__restrict__ cRb, int *__restrict__ pkey, int *__restrict__ ptgt,
-                               int *__restrict__ hist) {
+                               const double *__restrict__ cRb, int *__restrict__ pkey, int *__restrict__ hist) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= np) return;
-  int lo = 0, hi = nc;  // largest c with off[c] <= e
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const int t = lo, s = src[e];
-  ptgt[e] = t;
+  const int t = ptgt[e], s = src[e];
   int key = -1;
   // Same level or one level apart: then the offset's residue mod 4 fixes
   // the level relation (0: same, 2: source one up, odd: source one down).
   const double rt = cRb[t], rs = cRb[s];
   if (img[e] == 13 && (rs == rt || rs == 2 * rt || 2 * rs == rt)) {
-    const double u = 2 * rt / kLatUnit;
+    // Offsets in units of u = 2 R_t / kLatUnit; the lattice values are whole
+    // numbers to far better than the 1e-6 test below.
+    const double iu = kLatUnit / (2 * rt);
     const double4 a = cX[t], b = cX[s];
-    const double dx = (a.x - b.x) / u, dy = (a.y - b.y) / u, dz = (a.z - b.z) / u;
+    const double dx = (a.x - b.x) * iu, dy = (a.y - b.y) * iu, dz = (a.z - b.z) * iu;
     const double rx = rint(dx), ry = rint(dy), rz = rint(dz);
     if (fabs(dx - rx) < 1e-6 && fabs(dy - ry) < 1e-6 && fabs(dz - rz) < 1e-6 && fabs(rx) <= kLatD &&
         fabs(ry) <= kLatD && fabs(rz) <= kLatD)
@@ -148,8 +141,10 @@ struct TPad {
   static constexpr int TEP = NCH * CH;
 };
 
-// T(d) for every used offset, in fragment order: block (rb, kc) holds
-// T[rb*16 + l%16][kc*4 + l/16] at lane l.
+// T(d) of offset key okey[o] into row o of T, in fragment order: block
+// (rb, kc) holds T[rb*16 + l%16][kc*4 + l/16] at lane l.  T depends on the
+// key alone (offset in units of u and, through its residue, the level
+// relation), so rows are kept across evaluations.
 template <int P, typename R>
 __global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R *__restrict__ T) {
   using S = MfmaShape<P>;
@@ -261,10 +256,11 @@ __global__ void k_lat_mark(const int *__restrict__ ptgt, int64_t np, const int *
 }
 
 // Compact step list: alist[k] = offset of step k (steps of a tile in offset order).
+// Active step i = (tile, offset o): its T row, row_of_key[okey[o]].
 __global__ void k_lat_alist(const int *__restrict__ act, const int *__restrict__ aidx, int64_t n, int noff,
-                            int *__restrict__ alist) {
+                            const int *__restrict__ okey, const int *__restrict__ row_of_key, int *__restrict__ alist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && act[i]) alist[aidx[i]] = (int)(i % noff);
+  if (i < n && act[i]) alist[aidx[i]] = row_of_key[okey[i % noff]];
 }
 
 // Lattice pairs claim their (step, column) slot in three passes of plain
@@ -313,10 +309,25 @@ __global__ void k_lat_slots_src(int64_t n, const int *__restrict__ src, int *__r
   if (i < n && slot[i] >= 0) slot[i] = src[slot[i]];
 }
 
-// Offsets present in the list, numbered in key order.
-__global__ void k_lat_flag(const int *__restrict__ hist, int n, int *__restrict__ flag) {
+// Offsets present in the list (numbered in key order), and those of them
+// with no row in the T table yet.
+__global__ void k_lat_flag(const int *__restrict__ hist, const int *__restrict__ row_of_key, int n,
+                           int *__restrict__ flag, int *__restrict__ nflag) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) flag[k] = hist[k] != 0 ? 1 : 0;
+  if (k > n) return;
+  const bool used = k < n && hist[k] != 0;
+  flag[k] = used ? 1 : 0;
+  nflag[k] = used && row_of_key[k] < 0 ? 1 : 0;
+}
+
+// New rows base + scan[k] of the T table for the flagged keys.
+__global__ void k_tc_assign(const int *__restrict__ nflag, const int *__restrict__ scan, int n, int base,
+                            int *__restrict__ row_of_key, int *__restrict__ keys) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && nflag[k]) {
+    row_of_key[k] = base + scan[k];
+    keys[scan[k]] = k;
+  }
 }
 
 __global__ void k_lat_number(const int *__restrict__ flag, const int *__restrict__ scan, int n,
@@ -547,6 +558,34 @@ static void mscale_impl(Handle &h, hipStream_t st) {
   }
 }
 
+// An empty T table for expansion order P in FP32 or FP64.
+static void tc_reset(Handle &h, int P, bool f32) {
+  h.lat_tc_map.resize(kLatKeys + 1);
+  h.lat_tc_flag.resize(kLatKeys + 1);
+  h.lat_tc_scan.resize(kLatKeys + 1);
+  hipLaunchKernelGGL(k_fill_int, dim3(div_up(kLatKeys + 1, 256)), dim3(256), 0, h.stream, h.lat_tc_map.p,
+                     (int64_t)kLatKeys + 1, -1);
+  h.lat_tc_rows = 0;
+  h.lat_tc_P = P;
+  h.lat_tc_f32 = f32;
+}
+
+// Grow a buffer to `want` elements, keeping its first `keep`.
+template <typename R>
+static void grow_keep_rows(Handle &h, DBuf<R> &b, size_t want, size_t keep) {
+  if (want <= b.cap) return;
+  DBuf<R> nb;
+  nb.reserve(std::max(want, 2 * b.cap));
+  if (keep) EXA_HIP(hipMemcpyAsync(nb.p, b.p, keep * sizeof(R), hipMemcpyDeviceToDevice, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  b.release();
+  b.p = nb.p;
+  b.cap = nb.cap;
+  b.n = want;
+  nb.p = nullptr;
+  nb.cap = 0;
+}
+
 template <int P>
 static void lattice_run(Handle &h, M2LList &irr) {
   using S = MfmaShape<P>;
@@ -560,22 +599,23 @@ static void lattice_run(Handle &h, M2LList &irr) {
   if (!h.lat_ms_ready) mscale_impl<P>(h, h.stream);
   h.lat_ms_ready = false;
   // 1. Classify pairs (offset key, target cell) and flag the offsets used.
+  // (lat_ptgt: filled with the lists.)
   h.lat_pkey.resize(np > 0 ? np : 1);
-  h.lat_ptgt.resize(np > 0 ? np : 1);
   h.lat_hist.resize(kLatKeys);
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, (int64_t)kLatKeys,
                      0);
-  hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.m2l_src.p, h.m2l_img.p, np,
-                     h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_ptgt.p, h.lat_hist.p);
-  // 2. Offsets present, numbered in key order.
+  hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, h.m2l_src.p, h.m2l_img.p, np,
+                     h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_hist.p);
+  // 2. Offsets present, numbered in key order; offsets without a T row.
+  if (h.lat_tc_P != P || h.lat_tc_f32 != f32) tc_reset(h, P, f32);
   h.lat_flag.resize(kLatKeys + 1);
   h.lat_kscan.resize(kLatKeys + 1);
   h.lat_ookey.resize(kLatKeys);
   h.lat_okey.resize(kLatKeys);
-  hipLaunchKernelGGL(k_lat_flag, dim3(div_up(kLatKeys + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, kLatKeys,
-                     h.lat_flag.p);
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_flag.p + kLatKeys, (int64_t)1, 0);
+  hipLaunchKernelGGL(k_lat_flag, dim3(div_up(kLatKeys + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p,
+                     h.lat_tc_map.p, kLatKeys, h.lat_flag.p, h.lat_tc_flag.p);
   exclusive_scan_int(h, h.lat_flag.p, h.lat_kscan.p, kLatKeys + 1);
+  exclusive_scan_int(h, h.lat_tc_flag.p, h.lat_tc_scan.p, kLatKeys + 1);
   hipLaunchKernelGGL(k_lat_number, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_flag.p, h.lat_kscan.p,
                      kLatKeys, h.lat_ookey.p, h.lat_okey.p);
   // 3. Target tiles of one level and one octant (cell order inside a tile).
@@ -590,19 +630,36 @@ static void lattice_run(Handle &h, M2LList &irr) {
   const int *skey = alt ? tk + nc : tk, *sval = alt ? tv + nc : tv;
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, h.stream, h.lat_thist.p, (int64_t)256, 0);
   hipLaunchKernelGGL(k_tile_hist, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, h.lat_thist.p);
-  int ghist[256], noff = 0;
+  int ghist[256], noff = 0, nnew = 0;
   EXA_HIP(hipMemcpyAsync(ghist, h.lat_thist.p, sizeof(ghist), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipMemcpyAsync(&noff, h.lat_kscan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(&nnew, h.lat_tc_scan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
-  // T(d) of every offset: runs while the host lays out the tiles and steps.
-  if (noff > 0) {
+  if (h.lat_tc_rows + nnew > h.lat_tc_max) {  // start the table afresh
+    tc_reset(h, P, f32);
+    hipLaunchKernelGGL(k_lat_flag, dim3(div_up(kLatKeys + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p,
+                       h.lat_tc_map.p, kLatKeys, h.lat_flag.p, h.lat_tc_flag.p);
+    exclusive_scan_int(h, h.lat_tc_flag.p, h.lat_tc_scan.p, kLatKeys + 1);
+    nnew = noff;
+  }
+  // T(d) of the offsets not in the table yet: runs while the host lays out
+  // the tiles and steps.
+  if (nnew > 0) {
+    h.lat_tc_keys.resize(nnew);
+    hipLaunchKernelGGL(k_tc_assign, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_tc_flag.p,
+                       h.lat_tc_scan.p, kLatKeys, h.lat_tc_rows, h.lat_tc_map.p, h.lat_tc_keys.p);
     if (f32) {
-      h.lat_Tf.resize((size_t)noff * TPad<P, float>::TEP);
-      hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_Tf.p);
+      constexpr size_t TEP = TPad<P, float>::TEP;
+      grow_keep_rows(h, h.lat_Tf, (size_t)(h.lat_tc_rows + nnew) * TEP, (size_t)h.lat_tc_rows * TEP);
+      hipLaunchKernelGGL((k_lat_tmat<P, float>), dim3(nnew), dim3(64), 0, h.stream, h.lat_tc_keys.p,
+                         h.lat_Tf.p + (size_t)h.lat_tc_rows * TEP);
     } else {
-      h.lat_T.resize((size_t)noff * TPad<P, double>::TEP);
-      hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(noff), dim3(64), 0, h.stream, h.lat_okey.p, h.lat_T.p);
+      constexpr size_t TEP = TPad<P, double>::TEP;
+      grow_keep_rows(h, h.lat_T, (size_t)(h.lat_tc_rows + nnew) * TEP, (size_t)h.lat_tc_rows * TEP);
+      hipLaunchKernelGGL((k_lat_tmat<P, double>), dim3(nnew), dim3(64), 0, h.stream, h.lat_tc_keys.p,
+                         h.lat_T.p + (size_t)h.lat_tc_rows * TEP);
     }
+    h.lat_tc_rows += nnew;
   }
   int gb[512], ntiles = 0, start = 0;
   for (int g = 0; g < 256; g++) {
@@ -650,7 +707,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_step0.resize(ntiles + 1);
   if (nsteps_all) {
     hipLaunchKernelGGL(k_lat_alist, dim3(div_up(nsteps_all, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p, h.lat_aidx.p,
-                       nsteps_all, noff, h.lat_alist.p);
+                       nsteps_all, noff, h.lat_okey.p, h.lat_tc_map.p, h.lat_alist.p);
     // step0[t] = aidx[t * noff]: a strided copy.
     EXA_HIP(hipMemcpy2DAsync(h.lat_step0.p, sizeof(int), h.lat_aidx.p, (size_t)noff * sizeof(int), sizeof(int),
                              ntiles + 1, hipMemcpyDeviceToDevice, h.stream));

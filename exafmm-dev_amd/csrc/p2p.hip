@@ -148,8 +148,9 @@ struct TargetPasses {
 
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
-              const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
-              const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
+              const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+              const int *__restrict__ cNBody, const double4 *__restrict__ body, double4 *__restrict__ trg,
+              double cycle) {
   __shared__ double4 sb[kP2PStage];
   __shared__ double4 sacc[64];
   __shared__ SrcDesc sd;
@@ -230,12 +231,12 @@ __global__ void __launch_bounds__(64)
         __syncthreads();
         auto run = [&](auto mask) {
           for (int j = 0; j < cntp; j += 4 * GA) {
-  #pragma unroll
+#pragma unroll
             for (int u = 0; u < 4; u++) pair(mask, xa, sb[j + ga + u * GA], acca);
           }
           if (tp.r > 0) {
             for (int j = 0; j < cntp; j += 4 * GB) {
-  #pragma unroll
+#pragma unroll
               for (int u = 0; u < 4; u++) pair(mask, xb, sb[j + gb + u * GB], accb);
             }
           }
@@ -280,9 +281,9 @@ struct Kahan32 {
 
 __global__ void __launch_bounds__(64)
     k_p2p_mixed(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
-                const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody, const int *__restrict__ cNBody,
-                const double4 *__restrict__ cX, const double4 *__restrict__ body, double4 *__restrict__ trg,
-                double cycle) {
+                const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
+                double4 *__restrict__ trg, double cycle) {
   __shared__ float4 sb[kP2PStageF];   // hi parts and charge
   __shared__ float4 sbl[kP2PStageF];  // lo parts
   __shared__ double4 sacc[64];
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(64)
     // position skip R2 == 0 by a select.
     auto segment = [&](auto mask, const Tgt &t, int k0, int G, Acc &a) {
       float pot = 0, ax = 0, ay = 0, az = 0;
-  #pragma unroll
+#pragma unroll
       for (int u = 0; u < kSeg; u++) {
         const int k = k0 + u * G;
         const float4 xj = sb[k];

@@ -246,10 +246,11 @@ static void grow_keep(Handle &h, DBuf<T> &b, size_t want, size_t keep) {
 
 // Group (tgt[k], src[k], img[k]) by target: CSR offsets off[0..ncells], the
 // sources and images in target order (stable w.r.t. emission order), and the
-// compact list of targets with at least one entry.
+// compact list of targets with at least one entry; tgt_out (if given) gets
+// the target of every CSR entry.
 static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<unsigned char> &img,
                             DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist,
-                            int &ntl) {
+                            int &ntl, DBuf<int> *tgt_out = nullptr) {
   const int nc = h.ncells;
   const int T = 256;
   off.resize(nc + 1);
@@ -272,6 +273,10 @@ static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &sr
     keys = alt ? h.id.p : h.ic.p;
     hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, h.stream, order, np, src.p, img.p, src_out.p,
                        img_out.p);
+    if (tgt_out) {
+      tgt_out->resize(np);
+      EXA_HIP(hipMemcpyAsync(tgt_out->p, keys, np * sizeof(int), hipMemcpyDeviceToDevice, h.stream));
+    }
   }
   hipLaunchKernelGGL(k_csr_offsets, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, keys, np, nc, off.p);
   // Compact non-empty targets.
@@ -344,7 +349,8 @@ void run_lists(Handle &h) {
   }
   h.nm2l = nm;
   h.np2p = np;
-  group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt);
+  group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt,
+                  lattice_applies(h) ? &h.lat_ptgt : nullptr);
   group_by_target(h, np, pi, pj, pimg, h.p2p_off, h.p2p_src, h.p2p_img, h.p2p_tgt, h.np2p_tgt);
   EXA_HIP(hipGetLastError());
 }

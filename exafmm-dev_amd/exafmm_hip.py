@@ -133,6 +133,19 @@ class FMM:
         self.nterm = P * (P + 1) * (P + 2) // 6 if bas == 1 else P * (P + 1) // 2
         self.n = 0
 
+    def set_config(self, **changes):
+        """Change run-time parameters (theta, ncrit, precision, images, ...) as
+        exafmm_hip_set_config does; P, basis and device stay fixed."""
+        names = {"builder": {"uniform": 0, "adaptive": 1}, "precision": {"fp64": 0, "mixed": 1},
+                 "basis": {"spherical": 0, "cartesian": 1}}
+        for k, v in changes.items():
+            if k not in dict(Config._fields_):
+                raise KeyError(k)
+            if isinstance(v, str):
+                v = names[k][v]
+            setattr(self.cfg, k, v)
+        self._check(self.lib.exafmm_hip_set_config(self.h, ctypes.byref(self.cfg)), "set_config")
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.exafmm_hip_destroy(self.h)
