@@ -141,6 +141,19 @@ struct TPad {
   static constexpr int TEP = NCH * CH;
 };
 
+// Source multipole rows for the contraction: lane (column n, kq) of a wave
+// needs entries 4 kc + kq (kc = 0 .. KC-1) of its column's row.  A row keeps
+// them in groups of VL = 16 bytes / sizeof(R) consecutive kc per kq, the four
+// kq groups of one run of kc side by side: every 16-byte vector load of a
+// lane is one group, and the 4 lanes of a column read 64 contiguous bytes.
+// KC is padded to KCP (a multiple of VL) with zeros.
+template <int P, typename R>
+struct MsRow {
+  static constexpr int VL = 16 / (int)sizeof(R);
+  static constexpr int KCP = (MfmaShape<P>::KC + VL - 1) / VL * VL;
+  static constexpr int NKP = KCP * 4;  // entries per row
+};
+
 // T(d) of offset key okey[o] into row o of T, in fragment order: block
 // (rb, kc) holds T[rb*16 + l%16][kc*4 + l/16] at lane l.  T depends on the
 // key alone (offset in units of u and, through its residue, the level
@@ -372,17 +385,20 @@ __global__ void k_irr_compact(const int *__restrict__ irr, const int *__restrict
 
 // ---------------------------------------------------------------------------
 // Source multipoles scaled by their own cell's u^-n (u = w / kLatUnit), in
-// the precision of the contraction, KC * 4 entries per cell (zero padded);
-// row nc is all zero and stands for an empty column.
+// the precision of the contraction, as MsRow rows (zero padded); row nc is
+// all zero and stands for an empty column.
 template <int P, typename R>
 __global__ void __launch_bounds__(256)
     k_lat_mscale(int nc, const double *__restrict__ cRb, const cplx *__restrict__ M, R *__restrict__ Ms) {
   using S = MfmaShape<P>;
-  constexpr int NR = S::NR, NK = S::KC * 4;
+  using MR = MsRow<P, R>;
+  constexpr int NR = S::NR, NKP = MR::NKP, VL = MR::VL;
   constexpr DegreeTable<P> deg{};
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)(nc + 1) * NK) return;
-  const int c = (int)(i / NK), k = (int)(i % NK);
+  if (i >= (int64_t)(nc + 1) * NKP) return;
+  const int c = (int)(i / NKP), p = (int)(i % NKP);
+  const int r = p % (4 * VL), kq = r / VL, kc = (p / (4 * VL)) * VL + r % VL;
+  const int k = 4 * kc + kq;  // real row of the packed expansion (re, im interleaved)
   double v = 0;
   if (c < nc && k < NR) {
     const double iu = kLatUnit / (2 * cRb[c]);
@@ -424,8 +440,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
   using A = MfmaAcc<R>;
   using TP = TPad<P, R>;
   constexpr bool F32 = sizeof(R) == 4;
-  constexpr int RB = S::RB, KC = S::KC, NK = KC * 4;
-  constexpr int TEP = TP::TEP, CH = TP::CH, NCH = TP::NCH, VL = 16 / (int)sizeof(R);
+  using MR = MsRow<P, R>;
+  typedef R vec __attribute__((ext_vector_type(16 / sizeof(R))));
+  constexpr int RB = S::RB, KCP = MR::KCP, NKP = MR::NKP;
+  constexpr int TEP = TP::TEP, CH = TP::CH, NCH = TP::NCH, VL = MR::VL;
   __shared__ __attribute__((aligned(16))) R sT0[TEP];
   __shared__ __attribute__((aligned(16))) R sT1[TEP];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -439,9 +457,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
     const int i0 = a0 + (a1 - a0) * seg / kLatSeg, i1 = a0 + (a1 - a0) * (seg + 1) / kLatSeg;
     typename A::type acc[RB];
     v4d acc64[F32 ? RB : 1];
-  #pragma unroll
+#pragma unroll
     for (int rb = 0; rb < RB; rb++) acc[rb] = typename A::type{0, 0, 0, 0};
-  #pragma unroll
+#pragma unroll
     for (int rb = 0; rb < (F32 ? RB : 1); rb++) acc64[rb] = v4d{0, 0, 0, 0};
     // T(d) of step i into an LDS buffer: chunk ch by wave ch % 4.
     auto fetch_T = [&](R *dst, int i) {
@@ -449,8 +467,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
       for (int ch = wave; ch < NCH; ch += 4)
         __builtin_amdgcn_global_load_lds((glb_void *)(Tn + ch * CH), (lds_void *)(dst + ch * CH), 16, 0, 0);
     };
-    R b[KC];
+    R b[KCP];
     int sn = -1;
+    // Column of source cell `cell` (row nc: zeros) into v[0 .. KCP): one
+    // 16-byte load per VL entries (its kq group of the MsRow row).
+    auto load_col = [&](R *v, int cell) {
+      const vec *src = reinterpret_cast<const vec *>(Ms + (size_t)cell * NKP) + kq;
+#pragma unroll
+      for (int j = 0; j < KCP / VL; j++) {
+        const vec x = src[j * 4];
+#pragma unroll
+        for (int u = 0; u < VL; u++) v[j * VL + u] = x[u];
+      }
+    };
     // Step i from buffer CUR = (i - i0) & 1 while step i + 1 is fetched into
     // the other one.
     auto step = [&](auto curc, int i) {
@@ -458,23 +487,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
       const R *sTc = CUR ? sT1 : sT0;
       R *sTn = CUR ? sT0 : sT1;
       const bool more = i + 1 < i1;
-      R bn[KC];
+      R bn[KCP];
       int snn = -1;
       if (more) {
-        const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
-  #pragma unroll
-        for (int kc = 0; kc < KC; kc++) bn[kc] = src[kc * 4];
+        load_col(bn, sn < 0 ? nc : sn);
         fetch_T(sTn, i + 1);
         if (i + 2 < i1) snn = slot[(size_t)(i + 2) * 64 + col];
       }
-  #pragma unroll
+#pragma unroll
       for (int rb = 0; rb < RB; rb++) {
-  #pragma unroll
+#pragma unroll
         for (int kc = 0; kc < S::kc_end(rb); kc++)
           acc[rb] = A::mac(sTc[(S::base(rb) + kc) * 64 + lane], b[kc], acc[rb]);
       }
       if (F32) {
-  #pragma unroll
+#pragma unroll
         for (int rb = 0; rb < (F32 ? RB : 1); rb++)
           for (int r = 0; r < 4; r++) {
             acc64[rb][r] += (double)acc[rb][r];
@@ -482,8 +509,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
           }
       }
       if (more) {
-  #pragma unroll
-        for (int kc = 0; kc < KC; kc++) b[kc] = bn[kc];
+#pragma unroll
+        for (int kc = 0; kc < KCP; kc++) b[kc] = bn[kc];
         sn = snn;
       }
       __builtin_amdgcn_s_waitcnt(kVmcnt0);  // this wave's DMA and loads have landed
@@ -493,9 +520,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
     if (i0 < i1) {
       fetch_T(sT0, i0);
       sn = slot[(size_t)i0 * 64 + col];
-      const R *src = Ms + (size_t)(sn < 0 ? nc : sn) * NK + kq;
-  #pragma unroll
-      for (int kc = 0; kc < KC; kc++) b[kc] = src[kc * 4];
+      load_col(b, sn < 0 ? nc : sn);
       sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
     }
     __builtin_amdgcn_s_waitcnt(kVmcnt0);
@@ -505,9 +530,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
       if (i + 1 < i1) step(std::integral_constant<int, 1>{}, i + 1);
     }
     double *pp = part + (size_t)item * (RB * 16) * 64;
-  #pragma unroll
+#pragma unroll
     for (int rb = 0; rb < RB; rb++)
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = rb * 16 + d_row<R>(lane, r);
         const double v = F32 ? acc64[F32 ? rb : 0][r] : (double)acc[rb][r];
@@ -546,14 +571,15 @@ template <int P>
 static void mscale_impl(Handle &h, hipStream_t st) {
   using S = MfmaShape<P>;
   const int nc = h.ncells;
-  const int64_t nms = (int64_t)(nc + 1) * S::KC * 4;
+  const int64_t nms = (int64_t)(nc + 1) * MsRow<P, float>::NKP;
+  const int64_t nmd = (int64_t)(nc + 1) * MsRow<P, double>::NKP;
   if (h.cfg.precision == 1) {
     h.lat_Msf.resize(nms);
     hipLaunchKernelGGL((k_lat_mscale<P, float>), dim3(div_up(nms, 256)), dim3(256), 0, st, nc, h.cRb.p, h.M.p,
                        h.lat_Msf.p);
   } else {
-    h.lat_Ms.resize(nms);
-    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nms, 256)), dim3(256), 0, st, nc, h.cRb.p, h.M.p,
+    h.lat_Ms.resize(nmd);
+    hipLaunchKernelGGL((k_lat_mscale<P, double>), dim3(div_up(nmd, 256)), dim3(256), 0, st, nc, h.cRb.p, h.M.p,
                        h.lat_Ms.p);
   }
 }
