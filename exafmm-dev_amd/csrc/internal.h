@@ -76,6 +76,8 @@ struct Handle {
   hipEvent_t ev[18] = {};
   hipEvent_t ev_mult = nullptr;  // multipoles done on stream2 (split upward pass)
   bool mult_pending = false;     // the main stream has not joined ev_mult yet
+  hipEvent_t ev_cnt = nullptr;   // a device count copied into hpin
+  int *hpin = nullptr;           // pinned host words for counts read back behind queued work
 
   // Bodies.
   int n = 0;
@@ -146,6 +148,7 @@ struct Handle {
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
   DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
+  DBuf<int> lat_pslot;          // per-pair step, then slot (-1: not a lattice pair)
   // T(d) table kept across evaluations: offset key -> row of lat_T / lat_Tf
   // (-1: not computed yet), for expansion order lat_tc_P and FP32 or FP64.
   DBuf<int> lat_tc_map, lat_tc_flag, lat_tc_scan, lat_tc_keys;

@@ -28,6 +28,7 @@
 // not lattice pairs (levels two or more apart, periodic images, MASS mode,
 // offsets beyond kLatD) go to the per-pair kernel.
 #include <algorithm>
+#include <climits>
 #include <type_traits>
 #include <vector>
 
@@ -255,17 +256,24 @@ __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *_
   col_of[c] = col;
 }
 
-// Mark the (tile, offset) steps that have at least one pair.
+// Mark the (tile, offset) steps that have at least one pair; pstep[e] = the
+// pair's step tile * noff + o, or -1 (not a lattice pair).
 __global__ void k_lat_mark(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
                            const int *__restrict__ o_of_key, const int *__restrict__ tile_of, int noff,
-                           int *__restrict__ act) {
+                           int *__restrict__ act, int *__restrict__ pstep) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= np) return;
+  int v = -1;
   const int key = pkey[e];
   const int o = key >= 0 ? o_of_key[key] : -1;
-  if (o < 0) return;
-  const int tile = tile_of[ptgt[e]];
-  if (tile >= 0) act[(size_t)tile * noff + o] = 1;
+  if (o >= 0) {
+    const int tile = tile_of[ptgt[e]];
+    if (tile >= 0) {
+      v = tile * noff + o;
+      act[v] = 1;
+    }
+  }
+  pstep[e] = v;
 }
 
 // Compact step list: alist[k] = offset of step k (steps of a tile in offset order).
@@ -276,50 +284,44 @@ __global__ void k_lat_alist(const int *__restrict__ act, const int *__restrict__
   if (i < n && act[i]) alist[aidx[i]] = row_of_key[okey[i % noff]];
 }
 
-// Lattice pairs claim their (step, column) slot in three passes of plain
-// stores: (1) every lattice pair writes its own index into its slot, (2) a
-// pair whose slot holds another index (a repeated source: same target and
-// offset) and every non-lattice pair is irregular, counted per wave into
-// *nirr, (3) the slots' pair indices become source cells.
-__device__ inline int64_t lat_slot_of(int64_t e, const int *__restrict__ ptgt, const int *__restrict__ pkey,
-                                      const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
-                                      const int *__restrict__ col_of, int noff, const int *__restrict__ aidx) {
-  const int key = pkey[e];
-  const int o = key >= 0 ? o_of_key[key] : -1;
-  const int t = ptgt[e];
-  const int tile = tile_of[t];
-  if (o < 0 || tile < 0) return -1;
-  return (int64_t)aidx[(size_t)tile * noff + o] * 64 + col_of[t];
-}
-
-__global__ void k_lat_slots(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
-                            const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
-                            const int *__restrict__ col_of, int noff, const int *__restrict__ aidx,
-                            int *__restrict__ slot) {
+// Lattice pairs and their (step, column) slots, in two passes of plain
+// stores: (1) each lattice pair's slot aidx[step] * 64 + column (kept in
+// pslot[e]) takes the pair's index; (2) a pair whose slot holds another
+// index (a repeated source: same target and offset) and every non-lattice
+// pair is irregular, counted per wave into *nirr, and the pair a slot holds
+// writes its source cell s there as -2 - s (never equal to a pair index, so
+// the comparisons of other pairs stay right in any order).  Slots left at -1
+// are empty columns.
+__global__ void k_lat_slots(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ col_of,
+                            const int *__restrict__ aidx, int *__restrict__ pslot, int *__restrict__ slot) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= np) return;
-  const int64_t k = lat_slot_of(e, ptgt, pkey, o_of_key, tile_of, col_of, noff, aidx);
-  if (k >= 0) slot[k] = (int)e;
+  const int v = pslot[e];  // the step, from k_lat_mark
+  if (v < 0) return;
+  const int k = aidx[v] * 64 + col_of[ptgt[e]];
+  pslot[e] = k;
+  slot[k] = (int)e;
 }
 
-__global__ void k_lat_slots_check(const int *__restrict__ ptgt, int64_t np, const int *__restrict__ pkey,
-                                  const int *__restrict__ o_of_key, const int *__restrict__ tile_of,
-                                  const int *__restrict__ col_of, int noff, const int *__restrict__ aidx,
-                                  const int *__restrict__ slot, int *__restrict__ irr, int *__restrict__ nirr) {
+__global__ void k_lat_check(int64_t np, const int *__restrict__ pslot, const int *__restrict__ src,
+                            int *__restrict__ slot, int *__restrict__ irr, int *__restrict__ nirr) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool ir = false;
   if (e < np) {
-    const int64_t k = lat_slot_of(e, ptgt, pkey, o_of_key, tile_of, col_of, noff, aidx);
+    const int k = pslot[e];
     ir = k < 0 || slot[k] != (int)e;
+    if (!ir) slot[k] = -2 - src[e];
     irr[e] = ir ? 1 : 0;
   }
   const unsigned long long m = __ballot(ir);
   if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(nirr, (int)__popcll(m));
 }
 
-__global__ void k_lat_slots_src(int64_t n, const int *__restrict__ src, int *__restrict__ slot) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && slot[i] >= 0) slot[i] = src[slot[i]];
+// v into p[0 .. 64 * (*nsteps)) (the step count is on the device).
+__global__ void k_fill_slots(int *__restrict__ p, const int *__restrict__ nsteps, int v) {
+  const int64_t n = (int64_t)*nsteps * 64;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // Offsets present in the list (numbered in key order), and those of them
@@ -469,9 +471,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
     };
     R b[KCP];
     int sn = -1;
-    // Column of source cell `cell` (row nc: zeros) into v[0 .. KCP): one
-    // 16-byte load per VL entries (its kq group of the MsRow row).
-    auto load_col = [&](R *v, int cell) {
+    // Column of slot value sl (-2 - source cell, or -1: row nc of zeros)
+    // into v[0 .. KCP): one 16-byte load per VL entries (its kq group of the
+    // MsRow row).
+    auto load_col = [&](R *v, int sl) {
+      const int cell = sl <= -2 ? -2 - sl : nc;
       const vec *src = reinterpret_cast<const vec *>(Ms + (size_t)cell * NKP) + kq;
 #pragma unroll
       for (int j = 0; j < KCP / VL; j++) {
@@ -490,7 +494,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
       R bn[KCP];
       int snn = -1;
       if (more) {
-        load_col(bn, sn < 0 ? nc : sn);
+        load_col(bn, sn);
         fetch_T(sTn, i + 1);
         if (i + 2 < i1) snn = slot[(size_t)(i + 2) * 64 + col];
       }
@@ -520,7 +524,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
     if (i0 < i1) {
       fetch_T(sT0, i0);
       sn = slot[(size_t)i0 * 64 + col];
-      load_col(b, sn < 0 ? nc : sn);
+      load_col(b, sn);
       sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
     }
     __builtin_amdgcn_s_waitcnt(kVmcnt0);
@@ -583,6 +587,9 @@ static void mscale_impl(Handle &h, hipStream_t st) {
                        h.lat_Ms.p);
   }
 }
+
+// Largest slot array sized by the all-steps-active bound (512 MB of int).
+constexpr int64_t kSlotBound = (int64_t)1 << 27;
 
 // An empty T table for expansion order P in FP32 or FP64.
 static void tc_reset(Handle &h, int P, bool f32) {
@@ -722,13 +729,21 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_aidx.resize(nsteps_all + 1);
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(nsteps_all + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p,
                      nsteps_all + 1, 0);
-  if (nsteps_all)
-    hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
-                       h.lat_tile_of.p, noff, h.lat_act.p);
+  h.lat_pslot.resize(np);
+  hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
+                     h.lat_tile_of.p, noff, h.lat_act.p, h.lat_pslot.p);
   exclusive_scan_int(h, h.lat_act.p, h.lat_aidx.p, (int)(nsteps_all + 1));
-  int nact = 0;
-  EXA_HIP(hipMemcpyAsync(&nact, h.lat_aidx.p + nsteps_all, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipStreamSynchronize(h.stream));
+  // Step lists sized for every (tile, offset) being active, so the host need
+  // not wait for the active count (the device has it in aidx[nsteps_all]),
+  // unless that bound is large; then the count is read back.
+  int64_t nact = nsteps_all;
+  if (nsteps_all * 64 > kSlotBound) {
+    int na = 0;
+    EXA_HIP(hipMemcpyAsync(&na, h.lat_aidx.p + nsteps_all, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    nact = na;
+  }
+  if (nact * 64 > INT_MAX) throw Error(EXAFMM_HIP_EUNSUP, "lattice M2L: too many (tile, offset) steps");
   h.lat_alist.resize(nact > 0 ? nact : 1);
   h.lat_step0.resize(ntiles + 1);
   if (nsteps_all) {
@@ -739,25 +754,51 @@ static void lattice_run(Handle &h, M2LList &irr) {
                              ntiles + 1, hipMemcpyDeviceToDevice, h.stream));
   }
   // 5. Slots of the lattice pairs; the rest (and repeats) are irregular.
-  const int64_t nslot = (int64_t)nact * 64;
+  const int64_t nslot = nact * 64;
   h.lat_slot.resize(nslot > 0 ? nslot : 1);
   if (nslot)
-    hipLaunchKernelGGL(k_fill_int, dim3(div_up(nslot, TPB)), dim3(TPB), 0, h.stream, h.lat_slot.p, nslot, -1);
+    hipLaunchKernelGGL(k_fill_slots, dim3(std::min<int64_t>(div_up(nslot, TPB), 4096)), dim3(TPB), 0, h.stream,
+                       h.lat_slot.p, h.lat_aidx.p + nsteps_all, -1);
   h.lat_irr.resize(np + 1);
   h.lat_cnt.resize(1);
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_cnt.p, (int64_t)1, 0);
-  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
-                     h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p);
-  hipLaunchKernelGGL(k_lat_slots_check, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p,
-                     h.lat_ookey.p, h.lat_tile_of.p, h.lat_col_of.p, noff, h.lat_aidx.p, h.lat_slot.p, h.lat_irr.p,
-                     h.lat_cnt.p);
-  if (nslot)
-    hipLaunchKernelGGL(k_lat_slots_src, dim3(div_up(nslot, TPB)), dim3(TPB), 0, h.stream, nslot, h.m2l_src.p,
-                       h.lat_slot.p);
-  int nirr = 0;
-  EXA_HIP(hipMemcpyAsync(&nirr, h.lat_cnt.p, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipStreamSynchronize(h.stream));
-  // 6. The irregular pairs' CSR (empty for a uniform free-space tree).
+  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_col_of.p, h.lat_aidx.p,
+                     h.lat_pslot.p, h.lat_slot.p);
+  hipLaunchKernelGGL(k_lat_check, dim3(gp), dim3(TPB), 0, h.stream, np, h.lat_pslot.p, h.m2l_src.p, h.lat_slot.p,
+                     h.lat_irr.p, h.lat_cnt.p);
+  // The irregular count is read back behind the contraction's launch (pinned
+  // host word; the copy completes before the contraction starts).
+  EXA_HIP(hipMemcpyAsync(h.hpin, h.lat_cnt.p, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipEventRecord(h.ev_cnt, h.stream));
+  h.lat_stats_noff = noff;
+  h.lat_stats_tiles = ntiles;
+  // 6. The contraction (timed by ev[12..13]) and the reduction.
+  const bool run_mfma = noff > 0 && ntiles > 0 && nsteps_all > 0;
+  if (run_mfma) {
+    h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
+    // One workgroup per item, or lat_wg_cu workgroups per CU looping over
+    // the items (room left on every CU for the P2P waves of the other stream).
+    const int nitems = ntiles * kLatSeg;
+    int ngrid = nitems;
+    if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, h.lat_wg_cu * h.ncu);
+    const dim3 grid(ngrid);
+    EXA_HIP(hipEventRecord(h.ev[12], h.stream));
+    h.lat_timed = true;
+    if (f32)
+      hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
+                         h.lat_alist.p, h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
+    else
+      hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
+                         h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
+    EXA_HIP(hipEventRecord(h.ev[13], h.stream));
+    hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
+                       h.lat_part.p, h.L.p);
+    EXA_HIP(hipGetLastError());
+  }
+  EXA_HIP(hipEventSynchronize(h.ev_cnt));
+  const int nirr = *h.hpin;
+  // 7. The irregular pairs' CSR (empty for a uniform free-space tree),
+  // queued behind the contraction.
   int nitl = 0;
   h.irr_off.resize(nc + 1);
   if (nirr) {
@@ -782,28 +823,6 @@ static void lattice_run(Handle &h, M2LList &irr) {
   }
   irr = M2LList{h.irr_tgt.p, nitl, h.irr_off.p, h.irr_src.p, h.irr_img.p};
   h.lat_stats_pairs = np - nirr;
-  h.lat_stats_noff = noff;
-  h.lat_stats_tiles = ntiles;
-  // 7. The contraction (timed by ev[12..13]) and the reduction.
-  if (noff == 0 || ntiles == 0 || nact == 0) return;
-  h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
-  // One workgroup per item, or lat_wg_cu workgroups per CU looping over the
-  // items (room left on every CU for the P2P waves of the other stream).
-  const int nitems = ntiles * kLatSeg;
-  int ngrid = nitems;
-  if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, h.lat_wg_cu * h.ncu);
-  const dim3 grid(ngrid);
-  EXA_HIP(hipEventRecord(h.ev[12], h.stream));
-  h.lat_timed = true;
-  if (f32)
-    hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
-                       h.lat_alist.p, h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
-  else
-    hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
-                       h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
-  EXA_HIP(hipEventRecord(h.ev[13], h.stream));
-  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
-                     h.lat_part.p, h.L.p);
   EXA_HIP(hipGetLastError());
 }
 

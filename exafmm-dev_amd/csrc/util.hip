@@ -60,6 +60,8 @@ Handle::~Handle() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (ev_mult) (void)hipEventDestroy(ev_mult);
+  if (ev_cnt) (void)hipEventDestroy(ev_cnt);
+  if (hpin) (void)hipHostFree(hpin);
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
 }
