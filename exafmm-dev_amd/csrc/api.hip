@@ -167,7 +167,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   // EXAFMM_HIP_M2L=pairs keeps every M2L pair on the per-pair kernel.
   if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
-  if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_waves = std::atoi(m);
+  if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_wpc = std::atoi(m);  // per CU
   if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
   // Rows of the lattice T(d) table kept across evaluations (16384 rows are
   // 0.5 GB in FP64 at P = 10).

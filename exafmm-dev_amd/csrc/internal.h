@@ -134,13 +134,11 @@ struct Handle {
   bool lattice_m2l = true;
   // M2M / L2L as batched MFMA products with per-octant operators.
   bool updown_mfma = true, ud_ready = false;
-  // Co-residency of P2P (VALU) and the lattice M2L (matrix cores): P2P as a
-  // grid of p2p_waves waves per SIMD that claims target leaves from a
-  // counter (0 = one wave per leaf; -1 = 2 when the lattice M2L runs beside
-  // it, which leaves each CU room for one lattice workgroup, else 0), the
-  // lattice kernel as lat_wg_cu workgroups per CU looping over (tile,
-  // segment) items (0 = one workgroup per item).  ncu = compute units.
-  int p2p_waves = -1, lat_wg_cu = 0, ncu = 0;
+  // P2P as a resident grid of p2p_wpc waves per CU claiming target leaves
+  // from a counter (0: one wave per leaf); the lattice kernel as lat_wg_cu
+  // workgroups per CU looping over (tile, segment) items (0: one workgroup
+  // per item).  ncu = compute units.
+  int p2p_wpc = 12, lat_wg_cu = 0, ncu = 0;
   DBuf<double> ud_ops;
   DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
   DBuf<double> lat_part;

@@ -229,17 +229,33 @@ __global__ void __launch_bounds__(64)
         const int cntp = (cnt + g4 - 1) / g4 * g4;
         for (int k = cnt + lane; k < cntp; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
         __syncthreads();
-        auto run = [&](auto mask) {
-          for (int j = 0; j < cntp; j += 4 * GA) {
+        // One pass over the stage with G (1, 2, 4 or 8) a compile-time
+        // constant and a wave-uniform trip count: the source addresses of a
+        // group of 4 are one base and immediate offsets.
+        const int ncp = __builtin_amdgcn_readfirstlane(cntp);
+        auto pass_g = [&](auto mask, auto gc, const double4 &x, int g,
+                          double4 &acc) __attribute__((always_inline)) {
+          constexpr int G = decltype(gc)::value;
+          const double4 *sg = sb + g;
+          for (int j = 0; j < ncp; j += 4 * G) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) pair(mask, xa, sb[j + ga + u * GA], acca);
+            for (int u = 0; u < 4; u++) pair(mask, x, sg[j + u * G], acc);
           }
-          if (tp.r > 0) {
-            for (int j = 0; j < cntp; j += 4 * GB) {
-#pragma unroll
-              for (int u = 0; u < 4; u++) pair(mask, xb, sb[j + gb + u * GB], accb);
-            }
-          }
+        };
+        auto pass = [&](auto mask, int G, const double4 &x, int g,
+                        double4 &acc) __attribute__((always_inline)) {
+          if (G == 1)
+            pass_g(mask, std::integral_constant<int, 1>{}, x, g, acc);
+          else if (G == 2)
+            pass_g(mask, std::integral_constant<int, 2>{}, x, g, acc);
+          else if (G == 4)
+            pass_g(mask, std::integral_constant<int, 4>{}, x, g, acc);
+          else
+            pass_g(mask, std::integral_constant<int, 8>{}, x, g, acc);
+        };
+        auto run = [&](auto mask) __attribute__((always_inline)) {
+          pass(mask, GA, xa, ga, acca);
+          if (tp.r > 0) pass(mask, GB, xb, gb, accb);
         };
         if (__any(own))
           run(std::true_type{});
@@ -318,13 +334,15 @@ __global__ void __launch_bounds__(64)
     // kSeg sources k0, k0 + G, ... in FP32, then one compensated add each.
     // MASK as in k_p2p_f64: only stages that may hold the target's own
     // position skip R2 == 0 by a select.
-    auto segment = [&](auto mask, const Tgt &t, int k0, int G, Acc &a) {
+    auto segment = [&](auto mask, auto gc, const Tgt &t, int k0,
+                       Acc &a) __attribute__((always_inline)) {
+      constexpr int G = decltype(gc)::value;
+      const float4 *sg = sb + k0, *sgl = sbl + k0;
       float pot = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll
       for (int u = 0; u < kSeg; u++) {
-        const int k = k0 + u * G;
-        const float4 xj = sb[k];
-        const float4 xl = sbl[k];
+        const float4 xj = sg[u * G];
+        const float4 xl = sgl[u * G];
         const float dx = (t.x - xj.x) + (t.xl - xl.x), dy = (t.y - xj.y) + (t.yl - xl.y),
                     dz = (t.z - xj.z) + (t.zl - xl.z);
         const float r2 = dx * dx + dy * dy + dz * dz;
@@ -394,10 +412,28 @@ __global__ void __launch_bounds__(64)
           sbl[k] = make_float4(0, 0, 0, 0);
         }
         __syncthreads();
-        auto run = [&](auto mask) {
-          for (int j = 0; j < cntp; j += kSeg * GA) segment(mask, ta, j + ga, GA, acca);
-          if (tp.r > 0)
-            for (int j = 0; j < cntp; j += kSeg * GB) segment(mask, tb, j + gb, GB, accb);
+        // Passes with G (1, 2, 4 or 8) a compile-time constant and a
+        // wave-uniform trip count, as in k_p2p_f64.
+        const int ncp = __builtin_amdgcn_readfirstlane(cntp);
+        auto pass_g = [&](auto mask, auto gc, const Tgt &t, int g,
+                          Acc &a) __attribute__((always_inline)) {
+          constexpr int G = decltype(gc)::value;
+          for (int j = 0; j < ncp; j += kSeg * G) segment(mask, gc, t, j + g, a);
+        };
+        auto pass = [&](auto mask, int G, const Tgt &t, int g,
+                        Acc &a) __attribute__((always_inline)) {
+          if (G == 1)
+            pass_g(mask, std::integral_constant<int, 1>{}, t, g, a);
+          else if (G == 2)
+            pass_g(mask, std::integral_constant<int, 2>{}, t, g, a);
+          else if (G == 4)
+            pass_g(mask, std::integral_constant<int, 4>{}, t, g, a);
+          else
+            pass_g(mask, std::integral_constant<int, 8>{}, t, g, a);
+        };
+        auto run = [&](auto mask) __attribute__((always_inline)) {
+          pass(mask, GA, ta, ga, acca);
+          if (tp.r > 0) pass(mask, GB, tb, gb, accb);
         };
         if (__any(own))
           run(std::true_type{});
@@ -425,15 +461,13 @@ __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, co
 void run_p2p(Handle &h, hipStream_t st) {
   EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), st));
   if (h.np2p_tgt == 0) return;
-  // One wave per target leaf, or a grid of `waves` waves per SIMD (4 SIMDs
-  // per CU) that claims target leaves from a counter and leaves room on
-  // every CU: 2 P2P waves per SIMD (LDS 8 x 10.6 KB, at most 120 VGPRs each)
-  // beside one FP64 or FP32 lattice M2L workgroup (63.5 / 32 KB, 232 / 160
-  // VGPRs per wave) fit a CU's 160 KB of LDS and 512 VGPRs per SIMD lane.
-  int waves = h.p2p_waves;
-  if (waves < 0) waves = lattice_applies(h) && h.nm2l > 0 && h.cfg.P <= 10 ? 2 : 0;
+  // A resident grid of p2p_wpc waves per CU that claims target leaves from a
+  // counter (0: one wave per leaf).  A resident grid leaves the rest of each
+  // CU to the other stream's kernels (the lattice M2L set-up runs beside P2P
+  // instead of queueing behind one wave per leaf); 12 waves (10.6 KB of LDS
+  // and at most 128 VGPRs each) leave a third of the LDS and of the VGPRs.
   int grid = h.np2p_tgt;
-  if (waves > 0) grid = std::min(grid, waves * 4 * h.ncu);
+  if (h.p2p_wpc > 0) grid = std::min(grid, h.p2p_wpc * h.ncu);
   h.p2p_next.reserve(1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, h.p2p_next.p);
   if (h.cfg.precision == 1)
