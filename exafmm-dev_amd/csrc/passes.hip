@@ -167,20 +167,33 @@ __global__ void __launch_bounds__(64)
 
 // L2L (LaplaceSphericalCPU.cxx:193-224) for every cell of one level, then
 // L2P (:226-256) for the leaves among them.
-template <int P>
+//
+// PAIRS: L2P only, of the leaves in `cells` (all of them hold bodies), two
+// leaves per 64-lane workgroup: lanes 32 h .. 32 h + 31 take leaf
+// 2 blockIdx.x + h (a leaf holds about ncrit / 2 bodies, so a wave per leaf
+// would idle half of its lanes); used when the L2L ran level by level on the
+// matrix cores beforehand.  ncells: the number of entries of `cells`.
+template <int P, bool PAIRS>
 __global__ void __launch_bounds__(64)
     k_downward(const int *__restrict__ cells, const double4 *__restrict__ body, const double4 *__restrict__ cX,
                const int *__restrict__ cParent, const int *__restrict__ cNChild, const int *__restrict__ cBody,
                const int *__restrict__ cNBody, const cplx *__restrict__ M, cplx *__restrict__ L,
-               double4 *__restrict__ trg, int do_l2l, int mass) {
+               double4 *__restrict__ trg, int do_l2l, int mass, int ncells) {
   constexpr int NT = Expn<P>::NT;
-  __shared__ cplx sY[NT];
-  __shared__ cplx sL[NT];
-  __shared__ cplx sLc[NT];
-  const int c = cells[blockIdx.x];
-  const int lane = threadIdx.x;
+  constexpr int LW = PAIRS ? 32 : 64;  // lanes per cell
+  __shared__ cplx sY[PAIRS ? 1 : NT];
+  __shared__ cplx sL[PAIRS ? 1 : NT];
+  __shared__ cplx sLcs[PAIRS ? 2 : 1][NT];
+  const int half = PAIRS ? (int)threadIdx.x / 32 : 0;
+  const int lane = PAIRS ? (int)threadIdx.x % 32 : (int)threadIdx.x;
+  const int ci = PAIRS ? 2 * (int)blockIdx.x + half : (int)blockIdx.x;
+  if (PAIRS && 2 * (int)blockIdx.x + 1 >= ncells && half == 1) return;  // no second leaf (wave-uniform per half)
+  cplx *sLc = sLcs[half];
+  const int c = cells[ci];
   const double4 Xc = cX[c];
-  if (do_l2l) {
+  if (PAIRS) {
+    for (int t = lane; t < NT; t += LW) sLc[t] = L[(size_t)c * NT + t];
+  } else if (do_l2l) {
     const int p = cParent[c];
     const double4 Xp = cX[p];
     const Geom g = make_geom(Xc.x - Xp.x, Xc.y - Xp.y, Xc.z - Xp.z);
@@ -210,11 +223,13 @@ __global__ void __launch_bounds__(64)
       L[(size_t)c * NT + t] = own;
       sLc[t] = own;
     }
-  } else {
+  } else if (!PAIRS) {
     for (int t = lane; t < NT; t += 64) sLc[t] = L[(size_t)c * NT + t];
   }
-  if (cNChild[c] != 0) return;
-  __syncthreads();
+  if (!PAIRS && cNChild[c] != 0) return;
+  if (!PAIRS) __syncthreads();
+  // (PAIRS: each half's own lanes wrote its expansion; LDS stores and loads
+  // of one wave are ordered, so no barrier is needed between the halves.)
   // L2P: lanes own bodies.  The potential is sum_n sum_m w Re(L_n^m R_n^m)
   // (w = 1 for m = 0, else 2), as in LaplaceSphericalCPU.cxx:236-250.  The
   // gradient is taken in Cartesian form from the ladder relations of the
@@ -226,7 +241,7 @@ __global__ void __launch_bounds__(64)
   // 1/r and 1/sin(theta) factors, so it stays finite at the expansion centre
   // and on the polar axis, where cart2sph/sph2cart give NaN.
   const int b0 = cBody[c], nb = cNBody[c];
-  for (int bi = lane; bi < nb; bi += 64) {
+  for (int bi = lane; bi < nb; bi += LW) {
     const double4 v = body[b0 + bi];
     const Geom g = make_geom(v.x - Xc.x, v.y - Xc.y, v.z - Xc.z);
     const double rho = g.r, x = g.ct, y = g.st;
@@ -311,18 +326,29 @@ static void downward_impl(Handle &h) {
   // MASS mode divides each child's own L by its M_0 before the L2L
   // (LaplaceSphericalCPU.cxx MASS branch): kept in the per-cell kernel.
   const bool mf = updown_mfma_enabled(h) && !h.cfg.mass;
+  if (mf) {
+    // L2L level by level on the matrix cores, then L2P of every leaf with
+    // bodies (the P2P target list holds exactly those: a leaf is always in
+    // its own P2P list).
+    for (int t = 1; t < h.nlevels; t++) {
+      const int pc = h.lvl_off[t] - h.lvl_off[t - 1];
+      if (pc > 0 && h.lvl_off[t + 1] > h.lvl_off[t]) run_l2l_level_mfma(h, h.lvl_cells.p + h.lvl_off[t - 1], pc);
+    }
+    if (h.np2p_tgt > 0)
+      hipLaunchKernelGGL((k_downward<P, true>), dim3(div_up(h.np2p_tgt, 2)), dim3(64), 0, h.stream, h.p2p_tgt.p,
+                         h.body4.p, h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p,
+                         0, 0, h.np2p_tgt);
+    EXA_HIP(hipGetLastError());
+    return;
+  }
   for (int t = 0; t < h.nlevels; t++) {
     const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
     if (cnt == 0) continue;
     // Level 0 is the root: no L2L; L2P only when the root is a leaf
     // (up_down_pass.h:139).
-    if (mf && t > 0) {
-      const int pc = h.lvl_off[t] - h.lvl_off[t - 1];
-      if (pc > 0) run_l2l_level_mfma(h, h.lvl_cells.p + h.lvl_off[t - 1], pc);
-    }
-    hipLaunchKernelGGL(k_downward<P>, dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t], h.body4.p,
-                       h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p,
-                       t > 0 && !mf ? 1 : 0, h.cfg.mass);
+    hipLaunchKernelGGL((k_downward<P, false>), dim3(cnt), dim3(64), 0, h.stream, h.lvl_cells.p + h.lvl_off[t],
+                       h.body4.p, h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p,
+                       t > 0 ? 1 : 0, h.cfg.mass, cnt);
   }
   EXA_HIP(hipGetLastError());
 }
