@@ -557,7 +557,9 @@ __global__ void __launch_bounds__(256)
   constexpr DegreeTable<P> deg{};
   const int tile = blockIdx.x;
   double *Lr = reinterpret_cast<double *>(L);
-  for (int i = threadIdx.x; i < NR * 64; i += 256) {
+  // Rows 16 blockIdx.y .. 16 blockIdx.y + 15 of the tile.
+  const int r0 = 16 * blockIdx.y, r1 = min(NR, r0 + 16);
+  for (int i = r0 * 64 + threadIdx.x; i < r1 * 64; i += 256) {
     const int row = i / 64, col = i % 64;
     const int tgt = tile_tgt[tile * 64 + col];
     if (tgt < 0) continue;
@@ -791,7 +793,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
       hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
                          h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
     EXA_HIP(hipEventRecord(h.ev[13], h.stream));
-    hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
+    hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles, S::RB), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
                        h.lat_part.p, h.L.p);
     EXA_HIP(hipGetLastError());
   }
