@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--with-mixed", action="store_true", default=True)
     ap.add_argument("--no-mixed", dest="with_mixed", action="store_false")
+    ap.add_argument("--no-isolated-rates", dest="isolated_rates", action="store_false",
+                    help="skip the 3 extra evaluations that time the dominant kernels on an otherwise idle GPU")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--config", default="headline", choices=["headline", "batched", "plummer", "periodic"],
@@ -353,7 +355,38 @@ def main():
     ach = w / (t * 1e-3) / 1e12
     roofline = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 5), "traffic": measured_traffic(dom_name) if n == 1 << 20 and P == 10 else None,
-                "kernel": dom_name, "kernel_ms": round(t, 4)}
+                "kernel": dom_name, "kernel_ms": round(t, 4),
+                "note": ("kernel duration in the timed steps, where the contraction and P2P share the CUs "
+                         "(P2P: FP64 vector ALU, its peak equals the FP64 matrix peak)")}
+    # The same two kernels with the CUs to themselves: P2P as one wave per
+    # leaf (EXAFMM_HIP_P2P_WAVES=0) fills the GPU, so the contraction starts
+    # after it; 3 evaluations after the timed ones, outside the timed region.
+    if args.isolated_rates:
+        old_env = os.environ.get("EXAFMM_HIP_P2P_WAVES")
+        os.environ["EXAFMM_HIP_P2P_WAVES"] = "0"
+        try:
+            f = exafmm_hip.FMM(P=P, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, builder="uniform")
+            f.set_bodies(x)
+            f.evaluate()
+            iso = []
+            for _ in range(3):
+                f.evaluate()
+                iso.append(f.stats())
+            f.synchronize()
+            f.close()
+        finally:
+            if old_env is None:
+                del os.environ["EXAFMM_HIP_P2P_WAVES"]
+            else:
+                os.environ["EXAFMM_HIP_P2P_WAVES"] = old_env
+        iso_t = {"k_m2l_mfma<%d,double>" % P: float(np.median([d["t_m2l_mfma"] for d in iso])),
+                 "k_p2p_f64": float(np.median([d["t_p2p"] for d in iso]))}
+        roofline["isolated"] = {}
+        for k, ti in iso_t.items():
+            if ti > 0:
+                a = kern[k][0] / (ti * 1e-3) / 1e12
+                roofline["isolated"][k] = {"kernel_ms": round(ti, 4), "achieved": round(a, 4),
+                                           "frac": round(a / kern[k][3], 5)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, P)
     line = {
