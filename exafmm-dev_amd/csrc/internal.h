@@ -169,6 +169,10 @@ struct Handle {
   DBuf<uint64_t> k64a, k64b;
   DBuf<int> ia, ib, ic, id, ie, ifl;
   DBuf<int4> fa, fb;               // traversal frontiers (ci, cj, img, pad)
+  // Device-driven traversal: its state, the frontier capacity and the level
+  // count of the last traversal (0: run the per-level passes).
+  DBuf<char> trav_state;
+  int trav_fcap = 0, trav_levels = 0;
   DBuf<int> cnt;                   // small device counters
   DBuf<double> red;                // reduction scratch
 

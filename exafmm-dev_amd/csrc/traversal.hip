@@ -10,6 +10,8 @@
 // frontier visits the same set of pairs as the reference's TraverseRange
 // recursion (mutual or not); only the order differs, and the lists are
 // regrouped by target cell (CSR) afterwards.
+#include <algorithm>
+
 #include "internal.h"
 
 namespace exafmm {
@@ -71,10 +73,11 @@ __device__ inline int4 add4(int4 a, int4 b) { return make_int4(a.x + b.x, a.y + 
 
 constexpr int kTravBlock = 256;
 
-// Exclusive prefix of v over the 256 threads of the block (wave scans, then
+// Exclusive prefix of v over the NT threads of the block (wave scans, then
 // the wave totals); *total receives the block sum.
+template <int NT = kTravBlock>
 __device__ inline int4 block_exscan4(int4 v, int4 *total) {
-  __shared__ int4 wsum[kTravBlock / 64];
+  __shared__ int4 wsum[NT / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int4 inc = v;
   for (int d = 1; d < 64; d <<= 1) {
@@ -84,7 +87,7 @@ __device__ inline int4 block_exscan4(int4 v, int4 *total) {
   if (lane == 63) wsum[w] = inc;
   __syncthreads();
   int4 base = make_int4(0, 0, 0, 0), tot = make_int4(0, 0, 0, 0);
-  for (int k = 0; k < kTravBlock / 64; k++) {
+  for (int k = 0; k < NT / 64; k++) {
     if (k < w) base = add4(base, wsum[k]);
     tot = add4(tot, wsum[k]);
   }
@@ -111,25 +114,13 @@ __global__ void __launch_bounds__(kTravBlock)
   if (blockIdx.x == 0 && threadIdx.x == 0) blk[gridDim.x] = make_int4(0, 0, 0, 0);  // the scan ends in the totals
 }
 
-// Pass 2: each pair's output offsets = its block's offset (scan of the block
-// totals) + its prefix inside the block; the same order as one global
-// exclusive scan, so the lists are deterministic.
-__global__ void __launch_bounds__(kTravBlock)
-    k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
-           const int4 *__restrict__ blkofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i,
-           int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
-           int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
-  const int i = blockIdx.x * kTravBlock + threadIdx.x;
-  int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
-  int a = -1;
-  if (i < nf) {
-    f = fr[i];
-    a = act[i];
-    c = pair_counts(cv, f, a, mutual);
-  }
-  int4 tot;
-  const int4 ofs = add4(block_exscan4(c, &tot), blkofs[blockIdx.x]);
-  if (i >= nf) return;
+// The outputs of frontier pair f (action a) at offsets ofs (M2L, P2P, next
+// frontier) past the level's list bases.
+__device__ inline void emit_pair(const CellView &cv, int4 f, int a, int mutual, int4 ofs, int64_t base_m2l,
+                                 int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                                 unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+                                 int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img,
+                                 int4 *__restrict__ next) {
   const bool both = mutual && f.x != f.y;
   if (a == 0) {
     const int64_t o = base_m2l + ofs.x;
@@ -169,6 +160,253 @@ __global__ void __launch_bounds__(kTravBlock)
         for (int a1 = 0; a1 < nci; a1++)
           for (int b1 = 0; b1 < ncj; b1++) next[o++] = make_int4(ci0 + a1, cj0 + b1, f.z, 0);
       }
+    }
+  }
+}
+
+// Pass 2: each pair's output offsets = its block's offset (scan of the block
+// totals) + its prefix inside the block; the same order as one global
+// exclusive scan, so the lists are deterministic.
+__global__ void __launch_bounds__(kTravBlock)
+    k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
+           const int4 *__restrict__ blkofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i,
+           int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+           int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
+  const int i = blockIdx.x * kTravBlock + threadIdx.x;
+  int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
+  int a = -1;
+  if (i < nf) {
+    f = fr[i];
+    a = act[i];
+    c = pair_counts(cv, f, a, mutual);
+  }
+  int4 tot;
+  const int4 ofs = add4(block_exscan4(c, &tot), blkofs[blockIdx.x]);
+  if (i >= nf) return;
+  emit_pair(cv, f, a, mutual, ofs, base_m2l, base_p2p, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
+}
+
+// ---------------------------------------------------------------------------
+// The same passes driven from the device: the frontier size, the list
+// lengths and which frontier buffer is current live in a TravState on the
+// device, so the host enqueues several levels without waiting for any of
+// them.  A fixed grid of kTravGrid blocks takes contiguous chunks of the
+// frontier (multiples of 256 pairs, in order), so the emission order -- and
+// the lists -- are those of the per-level passes above.  Outputs that would
+// not fit the buffers set `overflow` and stop the traversal (the host then
+// runs the per-level passes, which grow the buffers).
+constexpr int kTravGrid = 512;
+
+struct TravState {
+  long long nm, np;              // list lengths so far
+  long long base_m2l, base_p2p;  // offsets of the level being emitted
+  int nf, nf_next, fsel;         // frontier size, next size, current buffer (0: fa, 1: fb)
+  int emit, overflow, levels;
+  int levels_small;              // levels done by k_trav_small
+  int done_blocks;               // k_emit_dev blocks finished (the last one advances the level)
+};
+
+constexpr int kSmallFrontier = 8192;  // levels up to this size run in one workgroup
+constexpr int kSmallBlock = 1024;
+
+__device__ inline void chunk_of(int nf, int &i0, int &i1) {
+  const int c = ((nf + kTravGrid - 1) / kTravGrid + kTravBlock - 1) / kTravBlock * kTravBlock;
+  i0 = min(nf, (int)blockIdx.x * c);
+  i1 = min(nf, i0 + c);
+}
+
+__global__ void k_trav_init(TravState *st, int nf) {
+  st->nm = st->np = st->base_m2l = st->base_p2p = 0;
+  st->nf = nf;
+  st->nf_next = 0;
+  st->fsel = 0;
+  st->emit = 0;
+  st->overflow = 0;
+  st->levels = 0;
+  st->levels_small = 0;
+  st->done_blocks = 0;
+}
+
+// The first levels, while 0 < nf <= kSmallFrontier, in one workgroup: both
+// passes of a level back to back (the frontier in 1024-pair chunks, in
+// order), no launch per level.
+__global__ void __launch_bounds__(kSmallBlock)
+    k_trav_small(CellView cv, int4 *__restrict__ fa, int4 *__restrict__ fb, TravState *st, double cycle,
+                 int nspawn, int mutual, int *__restrict__ act, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                 unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
+                 unsigned char *__restrict__ p2p_img, long long cap_m2l, long long cap_p2p, int cap_f) {
+  __shared__ int s_nf, s_fsel;
+  __shared__ long long s_nm, s_np;
+  if (threadIdx.x == 0) {
+    s_nf = st->nf;
+    s_fsel = st->fsel;
+    s_nm = st->nm;
+    s_np = st->np;
+  }
+  __syncthreads();
+  int levels = 0;
+  while (true) {
+    const int nf = s_nf;
+    if (nf == 0 || nf > kSmallFrontier) break;
+    const int4 *fr = s_fsel ? fb : fa;
+    int4 *next = s_fsel ? fa : fb;
+    const long long nm = s_nm, np = s_np;
+    int4 acc = make_int4(0, 0, 0, 0);
+    for (int s0 = 0; s0 < nf; s0 += kSmallBlock) {
+      const int i = s0 + threadIdx.x;
+      int4 c = make_int4(0, 0, 0, 0);
+      if (i < nf) {
+        const int4 f = fr[i];
+        const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
+        act[i] = a;
+        c = pair_counts(cv, f, a, mutual);
+      }
+      int4 tot;
+      block_exscan4<kSmallBlock>(c, &tot);
+      acc = add4(acc, tot);
+    }
+    if (!(nm + acc.x <= cap_m2l && np + acc.y <= cap_p2p && acc.z <= cap_f)) {
+      if (threadIdx.x == 0) st->overflow = 1;
+      break;
+    }
+    int4 run = make_int4(0, 0, 0, 0);
+    for (int s0 = 0; s0 < nf; s0 += kSmallBlock) {
+      const int i = s0 + threadIdx.x;
+      int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
+      int a = -1;
+      if (i < nf) {
+        f = fr[i];
+        a = act[i];
+        c = pair_counts(cv, f, a, mutual);
+      }
+      int4 tot;
+      const int4 ofs = add4(block_exscan4<kSmallBlock>(c, &tot), run);
+      run = add4(run, tot);
+      if (i < nf) emit_pair(cv, f, a, mutual, ofs, nm, np, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
+    }
+    levels++;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_nm = nm + acc.x;
+      s_np = np + acc.y;
+      s_nf = acc.z;
+      s_fsel ^= 1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (st->overflow) {
+      st->nf = 0;
+    } else {
+      st->nf = s_nf;
+      st->fsel = s_fsel;
+      st->nm = s_nm;
+      st->np = s_np;
+      st->levels += levels;
+      st->levels_small = levels;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTravBlock)
+    k_classify_dev(CellView cv, const int4 *__restrict__ fa, const int4 *__restrict__ fb, const TravState *st,
+                   double cycle, int nspawn, int mutual, int *__restrict__ act, int4 *__restrict__ blk) {
+  const int nf = st->nf;
+  const int4 *fr = st->fsel ? fb : fa;
+  int i0, i1;
+  chunk_of(nf, i0, i1);
+  int4 acc = make_int4(0, 0, 0, 0);
+  for (int s = i0; s < i1; s += kTravBlock) {
+    const int i = s + threadIdx.x;
+    int4 c = make_int4(0, 0, 0, 0);
+    if (i < i1) {
+      const int4 f = fr[i];
+      const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
+      act[i] = a;
+      c = pair_counts(cv, f, a, mutual);
+    }
+    int4 tot;
+    block_exscan4(c, &tot);
+    acc = add4(acc, tot);
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = acc;
+}
+
+// One block: exclusive scan of the kTravGrid block totals, the level's
+// output offsets and the capacity check.
+__global__ void __launch_bounds__(kTravBlock)
+    k_trav_scan(const int4 *__restrict__ blk, int4 *__restrict__ blkofs, TravState *st, long long cap_m2l,
+                long long cap_p2p, int cap_f) {
+  constexpr int PER = kTravGrid / kTravBlock;
+  int4 v[PER];
+  int4 sum = make_int4(0, 0, 0, 0);
+  for (int k = 0; k < PER; k++) {
+    v[k] = blk[threadIdx.x * PER + k];
+    sum = add4(sum, v[k]);
+  }
+  int4 tot;
+  int4 run = block_exscan4(sum, &tot);
+  for (int k = 0; k < PER; k++) {
+    blkofs[threadIdx.x * PER + k] = run;
+    run = add4(run, v[k]);
+  }
+  if (threadIdx.x == 0) {
+    const bool ok = st->nm + tot.x <= cap_m2l && st->np + tot.y <= cap_p2p && tot.z <= cap_f;
+    st->base_m2l = st->nm;
+    st->base_p2p = st->np;
+    st->emit = ok && st->nf > 0;
+    if (!ok) st->overflow = 1;
+    if (ok) {
+      st->nm += tot.x;
+      st->np += tot.y;
+      st->nf_next = tot.z;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTravBlock)
+    k_emit_dev(CellView cv, int4 *__restrict__ fa, int4 *__restrict__ fb, TravState *st, int mutual,
+               const int *__restrict__ act, const int4 *__restrict__ blkofs, int *__restrict__ m2l_i,
+               int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+               int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img) {
+  __shared__ int s_last;
+  const int emit = st->emit;
+  const int nf = st->nf;
+  const int fsel = st->fsel;
+  const int4 *fr = fsel ? fb : fa;
+  int4 *next = fsel ? fa : fb;
+  const long long bm = st->base_m2l, bp = st->base_p2p;
+  int i0, i1;
+  chunk_of(nf, i0, i1);
+  if (!emit) i1 = i0;
+  int4 run = blkofs[blockIdx.x];
+  for (int s = i0; s < i1; s += kTravBlock) {
+    const int i = s + threadIdx.x;
+    int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
+    int a = -1;
+    if (i < i1) {
+      f = fr[i];
+      a = act[i];
+      c = pair_counts(cv, f, a, mutual);
+    }
+    int4 tot;
+    const int4 ofs = add4(block_exscan4(c, &tot), run);
+    run = add4(run, tot);
+    if (i < i1) emit_pair(cv, f, a, mutual, ofs, bm, bp, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
+  }
+  // The last block to finish (every block has read the state by then) moves
+  // the state to the next level; the next kernel sees it.
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&st->done_blocks, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    st->done_blocks = 0;
+    if (!emit) {
+      st->nf = 0;
+    } else {
+      st->nf = st->nf_next;
+      st->fsel = fsel ^ 1;
+      st->levels++;
     }
   }
 }
@@ -290,6 +528,51 @@ static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &sr
   EXA_HIP(hipStreamSynchronize(h.stream));
 }
 
+// Levels of device-driven passes, enqueued without waiting; returns false
+// (nothing usable emitted) when an output did not fit its buffer or the
+// traversal did not end within kMaxLevels levels.  The frontier starts in fa.
+static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutual, int64_t &nm, int64_t &np) {
+  constexpr int kMaxLevels = 256;
+  // (fa and fb were sized by the caller before the frontier was written.)
+  h.ia.reserve(std::max(h.trav_fcap, nf0) + 1);
+  h.ib.reserve(4 * (size_t)kTravGrid);
+  h.ic.reserve(4 * (size_t)kTravGrid);
+  h.trav_state.reserve(sizeof(TravState));
+  TravState *st = reinterpret_cast<TravState *>(h.trav_state.p);
+  const long long cap_m = (long long)std::min(std::min(h.raw_mi.cap, h.raw_mj.cap), h.raw_mimg.cap) - 1;
+  const long long cap_p = (long long)std::min(std::min(h.raw_pi.cap, h.raw_pj.cap), h.raw_pimg.cap) - 1;
+  const int cap_fr = (int)std::min(std::min(h.fa.cap, h.fb.cap), h.ia.cap) - 1;  // frontier and action slots
+  int4 *blk = reinterpret_cast<int4 *>(h.ib.p), *blkofs = reinterpret_cast<int4 *>(h.ic.p);
+  hipLaunchKernelGGL(k_trav_init, dim3(1), dim3(1), 0, h.stream, st, nf0);
+  hipLaunchKernelGGL(k_trav_small, dim3(1), dim3(kSmallBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st, h.cfg.cycle,
+                     h.cfg.nspawn, mutual, h.ia.p, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
+                     h.raw_pimg.p, cap_m, cap_p, cap_fr);
+  TravState hs{};
+  int enq = 0;
+  int batch = h.trav_levels > 0 ? h.trav_levels + 1 : 8;  // levels after the small ones
+  while (true) {
+    for (int l = 0; l < batch; l++) {
+      hipLaunchKernelGGL(k_classify_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
+                         h.cfg.cycle, h.cfg.nspawn, mutual, h.ia.p, blk);
+      hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(kTravBlock), 0, h.stream, blk, blkofs, st, cap_m, cap_p, cap_fr);
+      hipLaunchKernelGGL(k_emit_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
+                         mutual, h.ia.p, blkofs, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
+                         h.raw_pimg.p);
+    }
+    enq += batch;
+    EXA_HIP(hipMemcpyAsync(&hs, st, sizeof(TravState), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    if (hs.overflow) return false;
+    if (hs.nf == 0) break;
+    if (enq >= kMaxLevels) return false;
+    batch = 4;
+  }
+  nm = hs.nm;
+  np = hs.np;
+  h.trav_levels = hs.levels - hs.levels_small;
+  return true;
+}
+
 void run_lists(Handle &h) {
   const int T = 256;
   CellView cv{h.cX.p, h.cChild.p, h.cNChild.p, h.cNBody.p};
@@ -300,15 +583,19 @@ void run_lists(Handle &h) {
   int nf = h.cfg.images == 0 ? 1 : 27;
   // Periodic images are traversed non-mutually (traversal.h:250-259).
   const int mutual = (h.cfg.mutual && h.cfg.images == 0) ? 1 : 0;
-  if (h.nsys > 1) {
-    nf = h.lvl_off[1] - h.lvl_off[0];  // the roots: level 0 of the forest
-    h.fa.reserve(nf + 64);
-    hipLaunchKernelGGL(k_init_frontier_roots, dim3(div_up(nf, 256)), dim3(256), 0, h.stream, h.fa.p,
-                       h.lvl_cells.p + h.lvl_off[0], nf);
-  } else {
-    h.fa.reserve(64);
-    hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
-  }
+  if (h.nsys > 1) nf = h.lvl_off[1] - h.lvl_off[0];  // the roots: level 0 of the forest
+  // Frontier buffers sized before the first frontier is written (a reserve
+  // that grows a buffer drops its contents).
+  h.fa.reserve(std::max(nf + 64, h.trav_fcap));
+  if (h.trav_fcap > 0) h.fb.reserve(std::max(nf + 64, h.trav_fcap));
+  auto init_frontier = [&]() {
+    if (h.nsys > 1)
+      hipLaunchKernelGGL(k_init_frontier_roots, dim3(div_up(nf, 256)), dim3(256), 0, h.stream, h.fa.p,
+                         h.lvl_cells.p + h.lvl_off[0], nf);
+    else
+      hipLaunchKernelGGL(k_init_frontier, dim3(1), dim3(64), 0, h.stream, h.fa.p, h.cfg.images);
+  };
+  init_frontier();
   // Start the raw lists at a capacity that covers uniform trees without
   // regrowth (about 180 M2L and 65 P2P sources per cell at theta = 0.4).
   grow_keep(h, mi, (size_t)h.ncells * 128 + 1024, 0);
@@ -318,7 +605,15 @@ void run_lists(Handle &h) {
   grow_keep(h, pj, (size_t)h.ncells * 48 + 1024, 0);
   grow_keep(h, pimg, (size_t)h.ncells * 48 + 1024, 0);
   h.cnt.reserve(4);
-  while (nf > 0) {
+  const int nf0 = nf;
+  bool done = false;
+  if (h.trav_fcap > 0) {
+    done = traverse_on_device(h, cv, nf0, mutual, nm, np);
+    if (!done) init_frontier();  // the per-level passes start over
+  }
+  if (!done) h.trav_levels = 0;
+  int max_nf = nf0;
+  while (!done && nf > 0) {
     const int nb = div_up(nf, kTravBlock);
     h.ia.reserve(nf + 1);  // action
     h.ib.reserve(4 * ((size_t)nb + 1));  // block totals (int4)
@@ -344,9 +639,12 @@ void run_lists(Handle &h) {
     nm += tot[0];
     np += tot[1];
     nf = tot[2];
+    max_nf = std::max(max_nf, nf);
+    h.trav_levels++;
     std::swap(h.fa.p, h.fb.p);
     std::swap(h.fa.cap, h.fb.cap);
   }
+  if (!done) h.trav_fcap = max_nf + max_nf / 4 + 64;  // the device-driven passes from the next call on
   h.nm2l = nm;
   h.np2p = np;
   group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt,

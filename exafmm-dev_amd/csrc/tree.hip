@@ -96,8 +96,10 @@ void run_bounds(Handle &h) {
 
 // getKey (build_tree_omp.h:36-57): float cell size promoted to double for
 // the division, truncation to int, bit interleave in int.
-__global__ void k_morton(const double4 *__restrict__ in, int n, double x0, double y0, double z0, float d, int level,
-                         uint64_t *__restrict__ key, int *__restrict__ idx) {
+// (The key has 3 * level <= 30 bits: it is kept as a 32-bit int for the
+// radix sort; k_widen_keys turns the sorted keys into the 64-bit form.)
+__global__ void k_morton32(const double4 *__restrict__ in, int n, double x0, double y0, double z0, float d, int level,
+                           int *__restrict__ key, int *__restrict__ idx) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const double4 v = in[b];
@@ -114,8 +116,13 @@ __global__ void k_morton(const double4 *__restrict__ in, int n, double x0, doubl
     iy >>= 1;
     iz >>= 1;
   }
-  key[b] = (uint64_t)(int64_t)id;
+  key[b] = id;
   idx[b] = b;
+}
+
+__global__ void k_widen_keys(const int *__restrict__ k32, int n, uint64_t *__restrict__ k64) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) k64[i] = (uint64_t)(int64_t)k32[i];
 }
 
 __global__ void k_gather_bodies(const double4 *__restrict__ in, const int *__restrict__ order, int n,
@@ -131,16 +138,10 @@ __global__ void k_gather_bodies(const double4 *__restrict__ in, const int *__res
 // prefix (key >> shift) differs from body i-1's.  flags has n+1 entries; the
 // last is 0 so that the exclusive scan yields the total at index n.
 // out[i] = in[i * stride], i < n (n small).
+constexpr int kMaxUniformLevels = 10;  // the reference's int keys hold 3 x 10 bits
+
 __global__ void k_strided_gather(const int *__restrict__ in, int64_t stride, int n, int *__restrict__ out) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = in[(int64_t)i * stride];
-}
-
-__global__ void k_level_heads(const uint64_t *__restrict__ key, int n, int shift, int *__restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  int f = 0;
-  if (i < n) f = (i == 0) || ((key[i] >> shift) != (key[i - 1] >> shift));
-  flags[i] = f;
 }
 
 __global__ void k_level_positions(const int *__restrict__ flags, const int *__restrict__ scan, int n,
@@ -148,6 +149,34 @@ __global__ void k_level_positions(const int *__restrict__ flags, const int *__re
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flags[i]) pos[scan[i]] = i;
+}
+
+// All levels at once (level t = blockIdx.y; arrays of stride n + 1 per level).
+__global__ void k_level_heads_all(const uint64_t *__restrict__ key, int n, int L, int64_t stride,
+                                  int *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+  if (i > n) return;
+  const int shift = 3 * (L - t);
+  int f = 0;
+  if (i < n) f = (i == 0) || ((key[i] >> shift) != (key[i - 1] >> shift));
+  flags[t * stride + i] = f;
+}
+
+// One exclusive scan ran over all levels: subtract each level's start.
+__global__ void k_level_rebase(int *__restrict__ scan, int n, int64_t stride, const int *__restrict__ base) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+  if (i <= n) scan[t * stride + i] -= base[t];
+}
+
+struct LevelOffsets {
+  int v[kMaxUniformLevels + 1];
+};
+
+__global__ void k_level_positions_all(const int *__restrict__ flags, const int *__restrict__ scan, int n,
+                                      int64_t stride, LevelOffsets posoff, int *__restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+  if (i >= n) return;
+  if (flags[t * stride + i]) pos[posoff.v[t] + scan[t * stride + i]] = i;
 }
 
 struct LevelArgs {
@@ -166,11 +195,11 @@ struct LevelArgs {
 // bodies2leafs / leafs2cells (build_tree_omp.h:130-198) fused with the index
 // arithmetic of reverseOrder: ascending rank r at level t lands at
 // off_t + count_t - 1 - r.
-__global__ void k_level_cells(LevelArgs a, const double4 *__restrict__ body, const uint64_t *__restrict__ key, int n,
-                              int nc, double x0, double y0, double z0, double4 *__restrict__ cX,
-                              double *__restrict__ cRb, int *__restrict__ cParent, int *__restrict__ cChild,
-                              int *__restrict__ cNChild, int *__restrict__ cBody, int *__restrict__ cNBody,
-                              uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
+__device__ inline void level_cell(const LevelArgs &a, const double4 *__restrict__ body,
+                                  const uint64_t *__restrict__ key, int n, int nc, double x0, double y0, double z0,
+                                  double4 *__restrict__ cX, double *__restrict__ cRb, int *__restrict__ cParent,
+                                  int *__restrict__ cChild, int *__restrict__ cNChild, int *__restrict__ cBody,
+                                  int *__restrict__ cNBody, uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.count) return;
   const int start = a.pos[r];
@@ -208,6 +237,20 @@ __global__ void k_level_cells(LevelArgs a, const double4 *__restrict__ body, con
   }
 }
 
+struct LevelArgsAll {
+  LevelArgs a[kMaxUniformLevels + 1];
+};
+
+// k_level_cells of every level in one launch (level t = blockIdx.y).
+__global__ void k_level_cells_all(LevelArgsAll all, const double4 *__restrict__ body, const uint64_t *__restrict__ key,
+                                  int n, int nc, double x0, double y0, double z0, double4 *__restrict__ cX,
+                                  double *__restrict__ cRb, int *__restrict__ cParent, int *__restrict__ cChild,
+                                  int *__restrict__ cNChild, int *__restrict__ cBody, int *__restrict__ cNBody,
+                                  uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
+  level_cell(all.a[blockIdx.y], body, key, n, nc, x0, y0, z0, cX, cRb, cParent, cChild, cNChild, cBody, cNBody, cKey,
+             cLevel);
+}
+
 __global__ void k_iota(int *__restrict__ a, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = i;
@@ -237,11 +280,16 @@ void run_build_uniform(Handle &h) {
   h.bidx.resize(n);
   h.trg4.resize(n);
   const int T = 256;
-  hipLaunchKernelGGL(k_morton, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.xmin[0], h.xmin[1], h.xmin[2], d,
-                     L, h.k64a.p, h.ia.p);
+  // Keys of 3 L <= 30 bits: sorted as 32-bit ints (a radix sort over 64-bit
+  // keys takes several times as many passes), then widened.  k64b holds the
+  // two 32-bit key buffers.
+  int *k32 = reinterpret_cast<int *>(h.k64b.p), *k32alt = k32 + (n + 1);
+  hipLaunchKernelGGL(k_morton32, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.xmin[0], h.xmin[1], h.xmin[2],
+                     d, L, k32, h.ia.p);
   bool alt = false;
-  sort_pairs_u64_int(h, h.k64a.p, h.k64b.p, h.ia.p, h.ib.p, n, 3 * L, alt);
-  uint64_t *key = alt ? h.k64b.p : h.k64a.p;
+  sort_pairs_int_int(h, k32, k32alt, h.ia.p, h.ib.p, n, 3 * L, alt);
+  hipLaunchKernelGGL(k_widen_keys, dim3(div_up(n, T)), dim3(T), 0, h.stream, alt ? k32alt : k32, n, h.k64a.p);
+  uint64_t *key = h.k64a.p;
   int *order = alt ? h.ib.p : h.ia.p;
   hipLaunchKernelGGL(k_gather_bodies, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, order, n, h.body4.p, h.bidx.p);
 
@@ -251,14 +299,17 @@ void run_build_uniform(Handle &h) {
   h.id.reserve(stride * (L + 1));  // scans
   std::vector<int> counts(L + 1);
   std::vector<int> tot(L + 1);
-  for (int t = 0; t <= L; t++) {
-    int *fl = h.ic.p + stride * t;
-    int *sc = h.id.p + stride * t;
-    hipLaunchKernelGGL(k_level_heads, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, key, n, 3 * (L - t), fl);
-    exclusive_scan_int(h, fl, sc, n + 1);
-  }
+  // Heads of every level, one exclusive scan over all levels, each level's
+  // start subtracted: the per-level exclusive scans.
+  h.cnt.reserve(2 * (L + 1));
+  hipLaunchKernelGGL(k_level_heads_all, dim3(div_up(n + 1, T), L + 1), dim3(T), 0, h.stream, key, n, L,
+                     (int64_t)stride, h.ic.p);
+  exclusive_scan_int(h, h.ic.p, h.id.p, (int)(stride * (L + 1)));
+  hipLaunchKernelGGL(k_strided_gather, dim3(1), dim3(64), 0, h.stream, h.id.p, (int64_t)stride, L + 1,
+                     h.cnt.p + L + 1);
+  hipLaunchKernelGGL(k_level_rebase, dim3(div_up(n + 1, T), L + 1), dim3(T), 0, h.stream, h.id.p, n, (int64_t)stride,
+                     h.cnt.p + L + 1);
   // Totals: scan[n] of each level, gathered on the device, one read-back.
-  h.cnt.reserve(L + 1);
   hipLaunchKernelGGL(k_strided_gather, dim3(1), dim3(64), 0, h.stream, h.id.p + n, (int64_t)stride, L + 1, h.cnt.p);
   EXA_HIP(hipMemcpyAsync(tot.data(), h.cnt.p, (L + 1) * sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
@@ -273,17 +324,16 @@ void run_build_uniform(Handle &h) {
   h.ncells = nc;
   h.nlevels = L + 1;
   h.ie.reserve(nc + 1);  // positions, level after level
-  std::vector<int> posoff(L + 1);
+  LevelOffsets posoff{};
   {
     int o = 0;
     for (int t = 0; t <= L; t++) {
-      posoff[t] = o;
+      posoff.v[t] = o;
       o += counts[t];
     }
   }
-  for (int t = 0; t <= L; t++)
-    hipLaunchKernelGGL(k_level_positions, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ic.p + stride * t,
-                       h.id.p + stride * t, n, h.ie.p + posoff[t]);
+  hipLaunchKernelGGL(k_level_positions_all, dim3(div_up(n, T), L + 1), dim3(T), 0, h.stream, h.ic.p, h.id.p, n,
+                     (int64_t)stride, posoff, h.ie.p);
   h.cX.resize(nc);
   h.cRb.resize(nc);
   h.cParent.resize(nc);
@@ -293,9 +343,11 @@ void run_build_uniform(Handle &h) {
   h.cNBody.resize(nc);
   h.cKey.resize(nc);
   h.cLevel.resize(nc);
+  LevelArgsAll all{};
+  int maxcount = 0;
   for (int t = 0; t <= L; t++) {
-    LevelArgs a;
-    a.pos = h.ie.p + posoff[t];
+    LevelArgs &a = all.a[t];
+    a.pos = h.ie.p + posoff.v[t];
     a.scan = h.id.p + stride * t;
     a.scan_up = t > 0 ? h.id.p + stride * (t - 1) : nullptr;
     a.scan_dn = t < L ? h.id.p + stride * (t + 1) : nullptr;
@@ -311,10 +363,11 @@ void run_build_uniform(Handle &h) {
     a.d = dt;
     a.leaf = (t == L);
     a.level = t;
-    hipLaunchKernelGGL(k_level_cells, dim3(div_up(counts[t], T)), dim3(T), 0, h.stream, a, h.body4.p, key, n, nc,
-                       h.xmin[0], h.xmin[1], h.xmin[2], h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p,
-                       h.cBody.p, h.cNBody.p, h.cKey.p, h.cLevel.p);
+    maxcount = std::max(maxcount, counts[t]);
   }
+  hipLaunchKernelGGL(k_level_cells_all, dim3(div_up(maxcount, T), L + 1), dim3(T), 0, h.stream, all, h.body4.p, key, n,
+                     nc, h.xmin[0], h.xmin[1], h.xmin[2], h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p,
+                     h.cBody.p, h.cNBody.p, h.cKey.p, h.cLevel.p);
   // Level order: the layout is already level-contiguous.
   h.lvl_off.assign(off.begin(), off.begin() + L + 2);
   h.lvl_cells.resize(nc);
