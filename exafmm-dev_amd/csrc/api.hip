@@ -181,6 +181,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));
     EXA_HIP(hipEventCreate(&H->h.ev_mult));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_cnt, hipEventDisableTiming));
+    EXA_HIP(hipEventCreateWithFlags(&H->h.ev_lists, hipEventDisableTiming));
     EXA_HIP(hipHostMalloc((void **)&H->h.hpin, 64 * sizeof(int), hipHostMallocDefault));
   } catch (const Error &e) {
     delete H;

@@ -77,6 +77,7 @@ struct Handle {
   hipEvent_t ev_mult = nullptr;  // multipoles done on stream2 (split upward pass)
   bool mult_pending = false;     // the main stream has not joined ev_mult yet
   hipEvent_t ev_cnt = nullptr;   // a device count copied into hpin
+  hipEvent_t ev_lists = nullptr; // raw lists done (the P2P grouping on stream2 waits for it)
   int *hpin = nullptr;           // pinned host words for counts read back behind queued work
 
   // Bodies.
@@ -166,6 +167,10 @@ struct Handle {
 
   // Scratch.
   DBuf<unsigned char> tmp;         // hipCUB temporary storage
+  // Scratch of the P2P list grouping, which runs on stream2 beside the M2L
+  // list grouping on the main stream.
+  DBuf<int> g2a, g2b, g2c, g2d, g2e;
+  DBuf<unsigned char> tmp2;
   DBuf<uint64_t> k64a, k64b;
   DBuf<int> ia, ib, ic, id, ie, ifl;
   DBuf<int4> fa, fb;               // traversal frontiers (ci, cj, img, pad)
@@ -226,10 +231,14 @@ int64_t count_p2p_body_pairs(Handle &h);
 
 // Scan / sort helpers (hipCUB) in util.hip.
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n);
+// The same on stream st with temporary storage tmp (concurrent users).
+void exclusive_scan_int(const int *in, int *out, int n, hipStream_t st, DBuf<unsigned char> &tmp);
 void exclusive_scan_int4(Handle &h, const int4 *in, int4 *out, int n);  // component-wise
 void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt);
 void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt);
+void sort_pairs_int_int(int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits, bool &result_in_alt,
+                        hipStream_t st, DBuf<unsigned char> &tmp);
 
 }  // namespace exafmm

@@ -206,7 +206,7 @@ struct TravState {
   int done_blocks;               // k_emit_dev blocks finished (the last one advances the level)
 };
 
-constexpr int kSmallFrontier = 8192;  // levels up to this size run in one workgroup
+constexpr int kSmallFrontier = 512;  // levels up to this size run in one workgroup
 constexpr int kSmallBlock = 1024;
 
 __device__ inline void chunk_of(int nf, int &i0, int &i1) {
@@ -482,50 +482,58 @@ static void grow_keep(Handle &h, DBuf<T> &b, size_t want, size_t keep) {
   nb.cap = 0;
 }
 
+// Scratch and stream of one list grouping (the M2L and P2P groupings run on
+// two streams at once).
+struct GroupCtx {
+  DBuf<int> &a, &b, &c, &d, &e;
+  DBuf<unsigned char> &tmp;
+  hipStream_t st;
+};
+
 // Group (tgt[k], src[k], img[k]) by target: CSR offsets off[0..ncells], the
 // sources and images in target order (stable w.r.t. emission order), and the
-// compact list of targets with at least one entry; tgt_out (if given) gets
-// the target of every CSR entry.
-static void group_by_target(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<unsigned char> &img,
-                            DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist,
-                            int &ntl, DBuf<int> *tgt_out = nullptr) {
+// compact list of targets with at least one entry, whose count is copied to
+// *ntl_host (pinned; valid after g.st is synchronised); tgt_out (if given)
+// gets the target of every CSR entry.  Nothing waits for the device.
+static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, DBuf<int> &src,
+                            DBuf<unsigned char> &img, DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out,
+                            DBuf<int> &tlist, int *ntl_host, DBuf<int> *tgt_out = nullptr) {
   const int nc = h.ncells;
   const int T = 256;
   off.resize(nc + 1);
   src_out.resize(np);
   img_out.resize(np);
-  h.ia.reserve(np + 1);
-  h.ib.reserve(np + 1);
-  h.ic.reserve(np + 1);
-  h.id.reserve(np + 1);
-  const int *keys = h.ic.p;
+  g.a.reserve(np + 1);
+  g.b.reserve(np + 1);
+  g.c.reserve(np + 1);
+  g.d.reserve(np + 1);
+  const int *keys = g.c.p;
   if (np > 0) {
-    hipLaunchKernelGGL(k_seq, dim3(div_up(np, T)), dim3(T), 0, h.stream, h.ia.p, np);
+    hipLaunchKernelGGL(k_seq, dim3(div_up(np, T)), dim3(T), 0, g.st, g.a.p, np);
     int bits = 1;
     while ((1 << bits) < nc) bits++;
     bool alt = false;
     // Keys: a copy of the targets (sorted stably), values: emission index.
-    EXA_HIP(hipMemcpyAsync(h.ic.p, tgt.p, np * sizeof(int), hipMemcpyDeviceToDevice, h.stream));
-    sort_pairs_int_int(h, h.ic.p, h.id.p, h.ia.p, h.ib.p, (int)np, bits, alt);
-    const int *order = alt ? h.ib.p : h.ia.p;
-    keys = alt ? h.id.p : h.ic.p;
-    hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, h.stream, order, np, src.p, img.p, src_out.p,
+    EXA_HIP(hipMemcpyAsync(g.c.p, tgt.p, np * sizeof(int), hipMemcpyDeviceToDevice, g.st));
+    sort_pairs_int_int(g.c.p, g.d.p, g.a.p, g.b.p, (int)np, bits, alt, g.st, g.tmp);
+    const int *order = alt ? g.b.p : g.a.p;
+    keys = alt ? g.d.p : g.c.p;
+    hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, g.st, order, np, src.p, img.p, src_out.p,
                        img_out.p);
     if (tgt_out) {
       tgt_out->resize(np);
-      EXA_HIP(hipMemcpyAsync(tgt_out->p, keys, np * sizeof(int), hipMemcpyDeviceToDevice, h.stream));
+      EXA_HIP(hipMemcpyAsync(tgt_out->p, keys, np * sizeof(int), hipMemcpyDeviceToDevice, g.st));
     }
   }
-  hipLaunchKernelGGL(k_csr_offsets, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, keys, np, nc, off.p);
+  hipLaunchKernelGGL(k_csr_offsets, dim3(div_up(nc + 1, T)), dim3(T), 0, g.st, keys, np, nc, off.p);
   // Compact non-empty targets.
-  h.ie.reserve(2 * (size_t)nc + 2);
-  int *flag = h.ie.p, *scan = h.ie.p + nc + 1;
-  hipLaunchKernelGGL(k_flag_nonzero, dim3(div_up(nc + 1, T)), dim3(T), 0, h.stream, off.p, nc, flag);
-  exclusive_scan_int(h, flag, scan, nc + 1);
+  g.e.reserve(2 * (size_t)nc + 2);
+  int *flag = g.e.p, *scan = g.e.p + nc + 1;
+  hipLaunchKernelGGL(k_flag_nonzero, dim3(div_up(nc + 1, T)), dim3(T), 0, g.st, off.p, nc, flag);
+  exclusive_scan_int(flag, scan, nc + 1, g.st, g.tmp);
   tlist.resize(nc);
-  hipLaunchKernelGGL(k_compact, dim3(div_up(nc, T)), dim3(T), 0, h.stream, flag, scan, nc, tlist.p);
-  EXA_HIP(hipMemcpyAsync(&ntl, scan + nc, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipStreamSynchronize(h.stream));
+  hipLaunchKernelGGL(k_compact, dim3(div_up(nc, T)), dim3(T), 0, g.st, flag, scan, nc, tlist.p);
+  EXA_HIP(hipMemcpyAsync(ntl_host, scan + nc, sizeof(int), hipMemcpyDeviceToHost, g.st));
 }
 
 // Levels of device-driven passes, enqueued without waiting; returns false
@@ -647,9 +655,18 @@ void run_lists(Handle &h) {
   if (!done) h.trav_fcap = max_nf + max_nf / 4 + 64;  // the device-driven passes from the next call on
   h.nm2l = nm;
   h.np2p = np;
-  group_by_target(h, nm, mi, mj, mimg, h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.nm2l_tgt,
-                  lattice_applies(h) ? &h.lat_ptgt : nullptr);
-  group_by_target(h, np, pi, pj, pimg, h.p2p_off, h.p2p_src, h.p2p_img, h.p2p_tgt, h.np2p_tgt);
+  // The two groupings at once: M2L on the main stream, P2P on the second
+  // stream (its own scratch), both waited for once.
+  EXA_HIP(hipEventRecord(h.ev_lists, h.stream));
+  EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev_lists, 0));
+  group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, nm, mi, mj, mimg, h.m2l_off, h.m2l_src,
+                  h.m2l_img, h.m2l_tgt, h.hpin + 8, lattice_applies(h) ? &h.lat_ptgt : nullptr);
+  group_by_target(h, GroupCtx{h.g2a, h.g2b, h.g2c, h.g2d, h.g2e, h.tmp2, h.stream2}, np, pi, pj, pimg, h.p2p_off,
+                  h.p2p_src, h.p2p_img, h.p2p_tgt, h.hpin + 9);
+  EXA_HIP(hipStreamSynchronize(h.stream2));
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  h.nm2l_tgt = h.hpin[8];
+  h.np2p_tgt = h.hpin[9];
   EXA_HIP(hipGetLastError());
 }
 

@@ -61,16 +61,19 @@ Handle::~Handle() {
     if (e) (void)hipEventDestroy(e);
   if (ev_mult) (void)hipEventDestroy(ev_mult);
   if (ev_cnt) (void)hipEventDestroy(ev_cnt);
+  if (ev_lists) (void)hipEventDestroy(ev_lists);
   if (hpin) (void)hipHostFree(hpin);
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
 }
 
-void exclusive_scan_int(Handle &h, const int *in, int *out, int n) {
+void exclusive_scan_int(Handle &h, const int *in, int *out, int n) { exclusive_scan_int(in, out, n, h.stream, h.tmp); }
+
+void exclusive_scan_int(const int *in, int *out, int n, hipStream_t st, DBuf<unsigned char> &tmp) {
   size_t bytes = 0;
-  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, h.stream));
-  h.tmp.reserve(bytes);
-  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(h.tmp.p, bytes, in, out, n, h.stream));
+  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
+  tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
 }
 
 struct Int4Sum {
@@ -100,12 +103,17 @@ void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals
 
 void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt) {
+  sort_pairs_int_int(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, h.stream, h.tmp);
+}
+
+void sort_pairs_int_int(int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits, bool &result_in_alt,
+                        hipStream_t st, DBuf<unsigned char> &tmp) {
   hipcub::DoubleBuffer<int> k(keys, keys_alt);
   hipcub::DoubleBuffer<int> v(vals, vals_alt);
   size_t bytes = 0;
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
-  h.tmp.reserve(bytes);
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(h.tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, st));
+  tmp.reserve(bytes);
+  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, st));
   result_in_alt = k.Current() != keys;
 }
 
