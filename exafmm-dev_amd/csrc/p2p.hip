@@ -33,7 +33,8 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
 }
 
 constexpr int kP2PStage = 256;  // source bodies staged in LDS per fill
-static_assert(kP2PStage % 32 == 0, "stages are padded to multiples of 4 G <= 32");
+constexpr int kP2PUnroll = 4;   // FP64 pairs in flight per lane
+static_assert(kP2PStage % (8 * kP2PUnroll) == 0, "stages are padded to multiples of kP2PUnroll G, G <= 8");
 constexpr double kFar = 1e100;  // position of the zero-charge padding bodies
 
 // 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
@@ -222,24 +223,25 @@ __global__ void __launch_bounds__(64)
           sb[k] = v;
           own |= (b >= tb0 && b < tb0 + tn) || im != 13;
         });
-        // Pad the stage to a multiple of 4 G (G of either pass, at most 8; the
-        // stage size is a multiple of 32) with zero charges far away, which
-        // add exactly zero, so both passes run whole groups of 4 sources.
-        const int g4 = 4 * (tp.r > 0 ? GB : GA);
+        // Pad the stage to a multiple of kP2PUnroll G (G of either pass, at
+        // most 8; the stage size is a multiple of 64) with zero charges far
+        // away, which add exactly zero, so both passes run whole groups of
+        // kP2PUnroll sources.
+        const int g4 = kP2PUnroll * (tp.r > 0 ? GB : GA);
         const int cntp = (cnt + g4 - 1) / g4 * g4;
         for (int k = cnt + lane; k < cntp; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
         __syncthreads();
         // One pass over the stage with G (1, 2, 4 or 8) a compile-time
         // constant and a wave-uniform trip count: the source addresses of a
-        // group of 4 are one base and immediate offsets.
+        // group are one base and immediate offsets.
         const int ncp = __builtin_amdgcn_readfirstlane(cntp);
         auto pass_g = [&](auto mask, auto gc, const double4 &x, int g,
                           double4 &acc) __attribute__((always_inline)) {
           constexpr int G = decltype(gc)::value;
           const double4 *sg = sb + g;
-          for (int j = 0; j < ncp; j += 4 * G) {
+          for (int j = 0; j < ncp; j += kP2PUnroll * G) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) pair(mask, x, sg[j + u * G], acc);
+            for (int u = 0; u < kP2PUnroll; u++) pair(mask, x, sg[j + u * G], acc);
           }
         };
         auto pass = [&](auto mask, int G, const double4 &x, int g,
