@@ -54,7 +54,7 @@ __global__ void k_active_flags(const int *__restrict__ nstart, const int *__rest
 
 __global__ void k_active_keys(const int *__restrict__ flag, const int *__restrict__ scan, int n,
                               const int *__restrict__ nstart, const double4 *__restrict__ nX, int nact,
-                              const double4 *__restrict__ body, int *__restrict__ apos, uint64_t *__restrict__ key,
+                              const double4 *__restrict__ body, int *__restrict__ apos, int *__restrict__ key,
                               int *__restrict__ val) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n || !flag[b]) return;
@@ -70,7 +70,7 @@ __global__ void k_active_keys(const int *__restrict__ flag, const int *__restric
   const int oct = (x.x > c.x) + ((x.y > c.y) << 1) + ((x.z > c.z) << 2);
   const int a = scan[b];
   apos[a] = b;
-  key[a] = ((uint64_t)lo << 3) | (uint64_t)oct;
+  key[a] = (lo << 3) | oct;  // node rank < 2^28
   val[a] = a;
 }
 
@@ -94,7 +94,7 @@ __global__ void k_copy_inactive(const int *__restrict__ flag, int n, const doubl
 
 // Child heads: position a starts a child when its (node, octant) key differs
 // from position a-1's.
-__global__ void k_child_heads(const uint64_t *__restrict__ skey, int na, int *__restrict__ head) {
+__global__ void k_child_heads(const int *__restrict__ skey, int na, int *__restrict__ head) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a > na) return;
   head[a] = a < na ? ((a == 0) || (skey[a] != skey[a - 1])) : 0;
@@ -116,7 +116,7 @@ __global__ void k_reset_parents(const int *__restrict__ agid, int nact, int *__r
 }
 
 struct ChildArgs {
-  const uint64_t *skey;
+  const int *skey;
   const int *apos;
   const int *hpos;        // sorted position of each child's first body
   int nch;                // children created at this level
@@ -137,8 +137,8 @@ __global__ void k_children(ChildArgs a) {
   if (c >= a.nch) return;
   const int i = a.hpos[c];
   const int e = c + 1 < a.nch ? a.hpos[c + 1] : a.na;
-  const uint64_t k = a.skey[i];
-  const int pr = (int)(k >> 3), oct = (int)(k & 7);
+  const int k = a.skey[i];
+  const int pr = k >> 3, oct = k & 7;
   const double4 P = a.nX[pr];
   double4 X;
   X.x = P.x + a.r * ((oct & 1) * 2 - 1);
@@ -383,17 +383,20 @@ void run_build_adaptive(Handle &h) {
     EXA_HIP(hipStreamSynchronize(h.stream));
     h.ie.reserve(na + 1);  // apos
     h.k64a.reserve(na + 1);
-    h.k64b.reserve(na + 1);
     h.ia.reserve(na + 1);
     h.ib.reserve(na + 1);
+    // Keys (node rank << 3 | octant) as 32-bit ints: the two key buffers in
+    // k64a (a radix sort over 64-bit keys takes several times as many passes).
+    if (nact >= (1 << 28)) throw Error(EXAFMM_HIP_EUNSUP, "adaptive tree: too many nodes on one level");
+    int *k32 = reinterpret_cast<int *>(h.k64a.p), *k32alt = k32 + (na + 1);
     hipLaunchKernelGGL(k_active_keys, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ifl.p, h.ic.p, n, astart.p, aX.p,
-                       nact, h.body4.p, h.ie.p, h.k64a.p, h.ia.p);
+                       nact, h.body4.p, h.ie.p, k32, h.ia.p);
     // 2. Stable sort by (node rank, octant).
     int bits = 3;
     while ((1LL << (bits - 3)) < nact) bits++;
     bool alt = false;
-    sort_pairs_u64_int(h, h.k64a.p, h.k64b.p, h.ia.p, h.ib.p, na, bits, alt);
-    const uint64_t *skey = alt ? h.k64b.p : h.k64a.p;
+    sort_pairs_int_int(h, k32, k32alt, h.ia.p, h.ib.p, na, bits, alt);
+    const int *skey = alt ? k32alt : k32;
     const int *order = alt ? h.ib.p : h.ia.p;
     // 3. Scatter back into the same positions.
     hipLaunchKernelGGL(k_copy_inactive, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ifl.p, n, h.body4.p, h.bidx.p,
