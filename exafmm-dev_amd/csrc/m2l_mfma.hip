@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
   // Items (tile, segment) = (item / 8, item % 8); a grid that is a multiple
   // of 8 keeps every item of a segment on one XCD.
   for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-    const int tile = item >> 3, seg = item & 7;
+    const int tile = item / kLatSeg, seg = item % kLatSeg;
     const int col = wave * 16 + (lane & 15);
     const int kq = lane >> 4;
     const int a0 = step0[tile], a1 = step0[tile + 1];
