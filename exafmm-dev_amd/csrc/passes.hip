@@ -388,13 +388,15 @@ void run_upward_split(Handle &h) {
   h.M.resize((size_t)h.ncells * h.nterm);
   h.L.resize((size_t)h.ncells * h.nterm);
   EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
+  // Radii first, alone (the traversal waits for them); the multipoles then
+  // run on the second stream beside the traversal.
+  EXA_DISPATCH_P3(upward_impl, h, 2, h.stream);
   EXA_HIP(hipEventRecord(h.ev[16], h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[16], 0));
   EXA_HIP(hipEventRecord(h.ev[17], h.stream2));
   EXA_DISPATCH_P3(upward_impl, h, 1, h.stream2);
   run_lattice_mscale(h, h.stream2);  // M2L input, ahead of P2P on this stream
   EXA_HIP(hipEventRecord(h.ev_mult, h.stream2));
-  EXA_DISPATCH_P3(upward_impl, h, 2, h.stream);
 }
 
 void run_downward(Handle &h) {
