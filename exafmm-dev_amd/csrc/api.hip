@@ -440,6 +440,7 @@ int exafmm_hip_get_lists(exafmm_hip_handle *H, int *p2p_i, int *p2p_j, int *m2l_
           for (int e = o[c]; e < o[c + 1]; e++) ti[e] = c;
     };
     expand(h.p2p_off, h.p2p_src, h.np2p, p2p_i, p2p_j);
+    ensure_m2l_grouped(h);
     expand(h.m2l_off, h.m2l_src, h.nm2l, m2l_i, m2l_j);
   });
 }

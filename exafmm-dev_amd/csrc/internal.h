@@ -133,6 +133,7 @@ struct Handle {
   // Lattice M2L (offset-grouped MFMA contractions) and the CSR of the
   // remaining (irregular) pairs.
   bool lattice_m2l = true;
+  bool m2l_grouped = false;  // m2l_off/src/img/tgt hold the M2L list grouped by target
   // M2M / L2L as batched MFMA products with per-octant operators.
   bool updown_mfma = true, ud_ready = false;
   // P2P as a resident grid of p2p_wpc waves per CU claiming target leaves
@@ -146,7 +147,10 @@ struct Handle {
   DBuf<int> lat_pkey, lat_hist, lat_flag, lat_kscan, lat_okey, lat_ookey, lat_tile_tgt, lat_tile_of, lat_col_of, lat_slot, lat_irr,
       lat_scan, irr_off, irr_src, irr_tgt;
   DBuf<unsigned char> irr_img;
-  DBuf<int> lat_ptgt, lat_cnt;  // per-pair target cell; irregular-pair counter
+  DBuf<int> lat_cnt;            // irregular-pair counter
+  DBuf<int> lat_has;            // cells with M2L pairs
+  DBuf<int> irr_rt, irr_rs;     // irregular pairs in emission order (target, source)
+  DBuf<unsigned char> irr_ri;   // ... and image
   DBuf<int> lat_pslot;          // per-pair step, then slot (-1: not a lattice pair)
   // T(d) table kept across evaluations: offset key -> row of lat_T / lat_Tf
   // (-1: not computed yet), for expansion order lat_tc_P and FP32 or FP64.
@@ -205,6 +209,12 @@ void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
 void run_upward_split(Handle &h);
 void run_lists(Handle &h);
+// The M2L list grouped by target (run_lists leaves it raw for the lattice M2L).
+void ensure_m2l_grouped(Handle &h);
+// Pairs (tgt, src, img) grouped by target into CSR (off, src_out, img_out)
+// and the targets with pairs (tlist, ntl); waits for the device.
+void group_pairs(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<unsigned char> &img, DBuf<int> &off,
+                 DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist, int &ntl);
 void run_p2p(Handle &h, hipStream_t st);
 void run_m2l(Handle &h);
 void run_downward(Handle &h);

@@ -212,18 +212,24 @@ __global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R
 
 // Target tiles: cells with M2L pairs keyed by (level, octant); a radix sort
 // on the key keeps cell order inside a group.
-__global__ void k_tile_key(int nc, const int *__restrict__ off, const int *__restrict__ lvl,
+__global__ void k_tile_key(int nc, const int *__restrict__ has, const int *__restrict__ lvl,
                            const uint64_t *__restrict__ ckey, int *__restrict__ key, int *__restrict__ val) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   int k = 255;
   const int l = lvl[c];
-  if (off[c + 1] > off[c] && l > 0 && l < 31) {
+  if (has[c] && l > 0 && l < 31) {
     const uint64_t lvloff = ((1ull << (3 * l)) - 1) / 7;
     k = l * 8 + (int)((ckey[c] - lvloff) & 7);
   }
   key[c] = k;
   val[c] = c;
+}
+
+// has[t] = 1 for every target t of the M2L list (plain stores of 1).
+__global__ void k_mark_has(const int *__restrict__ tgt, int64_t np, int *__restrict__ has) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < np) has[tgt[e]] = 1;
 }
 
 // Group sizes from the sorted keys: hist[g] = end - start of group g.
@@ -353,34 +359,20 @@ __global__ void k_lat_number(const int *__restrict__ flag, const int *__restrict
   if (flag[k]) okey[scan[k]] = k;
 }
 
-__global__ void k_flag_nonzero_off(const int *__restrict__ off, int nc, int *__restrict__ flag) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c <= nc) flag[c] = (c < nc && off[c + 1] > off[c]) ? 1 : 0;
-}
-
-__global__ void k_compact_idx(const int *__restrict__ flag, const int *__restrict__ scan, int n,
-                              int *__restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n && flag[c]) out[scan[c]] = c;
-}
-
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 // CSR of the irregular pairs: offsets from the exclusive scan of the flags.
-__global__ void k_irr_csr(const int *__restrict__ off, int nc, const int *__restrict__ scan,
-                          int *__restrict__ off_irr) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c <= nc) off_irr[c] = scan[off[c]];
-}
-
+// The irregular pairs (irr[e] = 1) in emission order.
 __global__ void k_irr_compact(const int *__restrict__ irr, const int *__restrict__ scan, int64_t np,
-                              const int *__restrict__ src, const unsigned char *__restrict__ img,
+                              const int *__restrict__ tgt, const int *__restrict__ src,
+                              const unsigned char *__restrict__ img, int *__restrict__ tgt_irr,
                               int *__restrict__ src_irr, unsigned char *__restrict__ img_irr) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= np || !irr[e]) return;
+  tgt_irr[scan[e]] = tgt[e];
   src_irr[scan[e]] = src[e];
   img_irr[scan[e]] = img[e];
 }
@@ -634,12 +626,14 @@ static void lattice_run(Handle &h, M2LList &irr) {
   if (!h.lat_ms_ready) mscale_impl<P>(h, h.stream);
   h.lat_ms_ready = false;
   // 1. Classify pairs (offset key, target cell) and flag the offsets used.
-  // (lat_ptgt: filled with the lists.)
+  // The raw (emission-order) M2L list: target, source, image per pair.
+  const int *ptgt = h.raw_mi.p, *psrc = h.raw_mj.p;
+  const unsigned char *pimg = h.raw_mimg.p;
   h.lat_pkey.resize(np > 0 ? np : 1);
   h.lat_hist.resize(kLatKeys);
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p, (int64_t)kLatKeys,
                      0);
-  hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, h.m2l_src.p, h.m2l_img.p, np,
+  hipLaunchKernelGGL(k_lat_classify, dim3(gp), dim3(TPB), 0, h.stream, ptgt, psrc, pimg, np,
                      h.cX.p, h.cRb.p, h.lat_pkey.p, h.lat_hist.p);
   // 2. Offsets present, numbered in key order; offsets without a T row.
   if (h.lat_tc_P != P || h.lat_tc_f32 != f32) tc_reset(h, P, f32);
@@ -658,7 +652,10 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_tval.resize(2 * (size_t)nc);
   h.lat_thist.resize(256);
   int *tk = h.lat_tkey.p, *tv = h.lat_tval.p;
-  hipLaunchKernelGGL(k_tile_key, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, h.m2l_off.p, h.cLevel.p, h.cKey.p,
+  h.lat_has.resize(nc);
+  hipLaunchKernelGGL(k_fill_int, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, h.lat_has.p, (int64_t)nc, 0);
+  hipLaunchKernelGGL(k_mark_has, dim3(gp), dim3(TPB), 0, h.stream, ptgt, np, h.lat_has.p);
+  hipLaunchKernelGGL(k_tile_key, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, h.lat_has.p, h.cLevel.p, h.cKey.p,
                      tk, tv);
   bool alt = false;
   sort_pairs_int_int(h, tk, tk + nc, tv, tv + nc, nc, 8, alt);
@@ -732,7 +729,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(nsteps_all + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_act.p,
                      nsteps_all + 1, 0);
   h.lat_pslot.resize(np);
-  hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_pkey.p, h.lat_ookey.p,
+  hipLaunchKernelGGL(k_lat_mark, dim3(gp), dim3(TPB), 0, h.stream, ptgt, np, h.lat_pkey.p, h.lat_ookey.p,
                      h.lat_tile_of.p, noff, h.lat_act.p, h.lat_pslot.p);
   exclusive_scan_int(h, h.lat_act.p, h.lat_aidx.p, (int)(nsteps_all + 1));
   // Step lists sized for every (tile, offset) being active, so the host need
@@ -764,9 +761,9 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_irr.resize(np + 1);
   h.lat_cnt.resize(1);
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_cnt.p, (int64_t)1, 0);
-  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, h.lat_ptgt.p, np, h.lat_col_of.p, h.lat_aidx.p,
+  hipLaunchKernelGGL(k_lat_slots, dim3(gp), dim3(TPB), 0, h.stream, ptgt, np, h.lat_col_of.p, h.lat_aidx.p,
                      h.lat_pslot.p, h.lat_slot.p);
-  hipLaunchKernelGGL(k_lat_check, dim3(gp), dim3(TPB), 0, h.stream, np, h.lat_pslot.p, h.m2l_src.p, h.lat_slot.p,
+  hipLaunchKernelGGL(k_lat_check, dim3(gp), dim3(TPB), 0, h.stream, np, h.lat_pslot.p, psrc, h.lat_slot.p,
                      h.lat_irr.p, h.lat_cnt.p);
   // The irregular count is read back behind the contraction's launch (pinned
   // host word; the copy completes before the contraction starts).
@@ -799,29 +796,20 @@ static void lattice_run(Handle &h, M2LList &irr) {
   }
   EXA_HIP(hipEventSynchronize(h.ev_cnt));
   const int nirr = *h.hpin;
-  // 7. The irregular pairs' CSR (empty for a uniform free-space tree),
-  // queued behind the contraction.
+  // 7. The irregular pairs (none for a uniform free-space tree), in
+  // emission order, grouped by target (a stable sort: within a target the
+  // order of the full list grouped by target), queued behind the contraction.
   int nitl = 0;
-  h.irr_off.resize(nc + 1);
   if (nirr) {
     h.lat_scan.resize(np + 1);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h.stream, h.lat_irr.p + np, (int64_t)1, 0);
     exclusive_scan_int(h, h.lat_irr.p, h.lat_scan.p, (int)(np + 1));
-    hipLaunchKernelGGL(k_irr_csr, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.m2l_off.p, nc, h.lat_scan.p,
-                       h.irr_off.p);
-    h.irr_src.resize(nirr);
-    h.irr_img.resize(nirr);
-    hipLaunchKernelGGL(k_irr_compact, dim3(gp), dim3(TPB), 0, h.stream, h.lat_irr.p, h.lat_scan.p, np, h.m2l_src.p,
-                       h.m2l_img.p, h.irr_src.p, h.irr_img.p);
-    // Targets with irregular pairs.
-    h.ie.reserve(2 * (size_t)nc + 2);
-    int *flag = h.ie.p, *scan = h.ie.p + nc + 1;
-    hipLaunchKernelGGL(k_flag_nonzero_off, dim3(div_up(nc + 1, TPB)), dim3(TPB), 0, h.stream, h.irr_off.p, nc, flag);
-    exclusive_scan_int(h, flag, scan, nc + 1);
-    h.irr_tgt.resize(nc);
-    hipLaunchKernelGGL(k_compact_idx, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, flag, scan, nc, h.irr_tgt.p);
-    EXA_HIP(hipMemcpyAsync(&nitl, scan + nc, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipStreamSynchronize(h.stream));
+    h.irr_rt.resize(nirr);
+    h.irr_rs.resize(nirr);
+    h.irr_ri.resize(nirr);
+    hipLaunchKernelGGL(k_irr_compact, dim3(gp), dim3(TPB), 0, h.stream, h.lat_irr.p, h.lat_scan.p, np, ptgt, psrc,
+                       pimg, h.irr_rt.p, h.irr_rs.p, h.irr_ri.p);
+    group_pairs(h, nirr, h.irr_rt, h.irr_rs, h.irr_ri, h.irr_off, h.irr_src, h.irr_img, h.irr_tgt, nitl);
   }
   irr = M2LList{h.irr_tgt.p, nitl, h.irr_off.p, h.irr_src.p, h.irr_img.p};
   h.lat_stats_pairs = np - nirr;

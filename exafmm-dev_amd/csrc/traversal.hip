@@ -493,11 +493,11 @@ struct GroupCtx {
 // Group (tgt[k], src[k], img[k]) by target: CSR offsets off[0..ncells], the
 // sources and images in target order (stable w.r.t. emission order), and the
 // compact list of targets with at least one entry, whose count is copied to
-// *ntl_host (pinned; valid after g.st is synchronised); tgt_out (if given)
-// gets the target of every CSR entry.  Nothing waits for the device.
+// *ntl_host (pinned; valid after g.st is synchronised).  Nothing waits for
+// the device.
 static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, DBuf<int> &src,
                             DBuf<unsigned char> &img, DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out,
-                            DBuf<int> &tlist, int *ntl_host, DBuf<int> *tgt_out = nullptr) {
+                            DBuf<int> &tlist, int *ntl_host) {
   const int nc = h.ncells;
   const int T = 256;
   off.resize(nc + 1);
@@ -520,10 +520,6 @@ static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, D
     keys = alt ? g.d.p : g.c.p;
     hipLaunchKernelGGL(k_permute_pairs, dim3(div_up(np, T)), dim3(T), 0, g.st, order, np, src.p, img.p, src_out.p,
                        img_out.p);
-    if (tgt_out) {
-      tgt_out->resize(np);
-      EXA_HIP(hipMemcpyAsync(tgt_out->p, keys, np * sizeof(int), hipMemcpyDeviceToDevice, g.st));
-    }
   }
   hipLaunchKernelGGL(k_csr_offsets, dim3(div_up(nc + 1, T)), dim3(T), 0, g.st, keys, np, nc, off.p);
   // Compact non-empty targets.
@@ -655,19 +651,40 @@ void run_lists(Handle &h) {
   if (!done) h.trav_fcap = max_nf + max_nf / 4 + 64;  // the device-driven passes from the next call on
   h.nm2l = nm;
   h.np2p = np;
-  // The two groupings at once: M2L on the main stream, P2P on the second
-  // stream (its own scratch), both waited for once.
+  // The lattice M2L works on the raw (emission-order) M2L list; the M2L
+  // list is grouped by target only for the other M2L paths (and on request,
+  // `ensure_m2l_grouped`).  With both grouped, the two groupings run at once:
+  // M2L on the main stream, P2P on the second stream (its own scratch).
+  h.m2l_grouped = !lattice_applies(h);
   EXA_HIP(hipEventRecord(h.ev_lists, h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev_lists, 0));
-  group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, nm, mi, mj, mimg, h.m2l_off, h.m2l_src,
-                  h.m2l_img, h.m2l_tgt, h.hpin + 8, lattice_applies(h) ? &h.lat_ptgt : nullptr);
+  if (h.m2l_grouped)
+    group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, nm, mi, mj, mimg, h.m2l_off,
+                    h.m2l_src, h.m2l_img, h.m2l_tgt, h.hpin + 8);
   group_by_target(h, GroupCtx{h.g2a, h.g2b, h.g2c, h.g2d, h.g2e, h.tmp2, h.stream2}, np, pi, pj, pimg, h.p2p_off,
                   h.p2p_src, h.p2p_img, h.p2p_tgt, h.hpin + 9);
   EXA_HIP(hipStreamSynchronize(h.stream2));
   EXA_HIP(hipStreamSynchronize(h.stream));
-  h.nm2l_tgt = h.hpin[8];
+  h.nm2l_tgt = h.m2l_grouped ? h.hpin[8] : 0;
   h.np2p_tgt = h.hpin[9];
   EXA_HIP(hipGetLastError());
+}
+
+void ensure_m2l_grouped(Handle &h) {
+  if (h.m2l_grouped) return;
+  group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, h.nm2l, h.raw_mi, h.raw_mj, h.raw_mimg,
+                  h.m2l_off, h.m2l_src, h.m2l_img, h.m2l_tgt, h.hpin + 8);
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  h.nm2l_tgt = h.hpin[8];
+  h.m2l_grouped = true;
+}
+
+void group_pairs(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<unsigned char> &img, DBuf<int> &off,
+                 DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist, int &ntl) {
+  group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, np, tgt, src, img, off, src_out, img_out,
+                  tlist, h.hpin + 10);
+  EXA_HIP(hipStreamSynchronize(h.stream));
+  ntl = h.hpin[10];
 }
 
 }  // namespace exafmm
