@@ -97,6 +97,7 @@ static void do_traverse(Handle &h) {
   ensure(h.have_up, "traverse: upward pass not run");
   EXA_HIP(hipEventRecord(h.ev[4], h.stream));
   run_lists(h);
+  enqueue_multipoles(h);  // if the traversal did not
   EXA_HIP(hipEventRecord(h.ev[5], h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[5], 0));
   EXA_HIP(hipEventRecord(h.ev[9], h.stream2));
@@ -292,9 +293,10 @@ static void evaluate_all(Handle &h) {
     const bool split = h.cfg.basis == 0 && !h.cfg.useRopt;
     if (split) {
       ensure(h.have_tree, "upward_pass: tree not built");
-      run_upward_split(h);
+      run_upward_radii(h);
       h.have_up = true;
       h.mult_pending = true;
+      h.mult_enqueue = true;
     } else {
       do_upward(h);
     }

@@ -76,6 +76,7 @@ struct Handle {
   hipEvent_t ev[18] = {};
   hipEvent_t ev_mult = nullptr;  // multipoles done on stream2 (split upward pass)
   bool mult_pending = false;     // the main stream has not joined ev_mult yet
+  bool mult_enqueue = false;     // run_upward_mult not enqueued yet (the traversal does it)
   hipEvent_t ev_cnt = nullptr;   // a device count copied into hpin
   hipEvent_t ev_lists = nullptr; // raw lists done (the P2P grouping on stream2 waits for it)
   int *hpin = nullptr;           // pinned host words for counts read back behind queued work
@@ -207,7 +208,14 @@ void run_build_uniform(Handle &h);
 void run_build_forest(Handle &h);
 void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
-void run_upward_split(Handle &h);
+void run_upward_radii(Handle &h);  // split upward pass: radii on the main stream
+void run_upward_mult(Handle &h);   // ... multipoles on the second stream after them
+// run_upward_mult once, if the split upward pass left it to be enqueued.
+inline void enqueue_multipoles(Handle &h) {
+  if (!h.mult_enqueue) return;
+  h.mult_enqueue = false;
+  run_upward_mult(h);
+}
 void run_lists(Handle &h);
 // The M2L list grouped by target (run_lists leaves it raw for the lattice M2L).
 void ensure_m2l_grouped(Handle &h);

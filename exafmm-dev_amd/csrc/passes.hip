@@ -383,15 +383,19 @@ void run_upward(Handle &h) {
 
 // The upward pass split over the two streams: radii (needed by the
 // traversal) on the main stream, multipoles (needed only by M2L) on the
-// second stream, which the caller joins through h.ev_mult before M is read.
-void run_upward_split(Handle &h) {
+// second stream after the radii, joined by the caller through h.ev_mult
+// before M is read.  run_upward_mult is enqueued by the traversal once its
+// own first kernels are queued (its launches would otherwise hold the host
+// while the main stream idles).
+void run_upward_radii(Handle &h) {
   h.M.resize((size_t)h.ncells * h.nterm);
   h.L.resize((size_t)h.ncells * h.nterm);
   EXA_HIP(hipMemsetAsync(h.L.p, 0, h.L.bytes(), h.stream));
-  // Radii first, alone (the traversal waits for them); the multipoles then
-  // run on the second stream beside the traversal.
   EXA_DISPATCH_P3(upward_impl, h, 2, h.stream);
   EXA_HIP(hipEventRecord(h.ev[16], h.stream));
+}
+
+void run_upward_mult(Handle &h) {
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[16], 0));
   EXA_HIP(hipEventRecord(h.ev[17], h.stream2));
   EXA_DISPATCH_P3(upward_impl, h, 1, h.stream2);

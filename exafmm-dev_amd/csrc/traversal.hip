@@ -564,6 +564,7 @@ static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutua
                          h.raw_pimg.p);
     }
     enq += batch;
+    enqueue_multipoles(h);  // the second stream's work, while these levels run
     EXA_HIP(hipMemcpyAsync(&hs, st, sizeof(TravState), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
     if (hs.overflow) return false;
@@ -616,6 +617,7 @@ void run_lists(Handle &h) {
     if (!done) init_frontier();  // the per-level passes start over
   }
   if (!done) h.trav_levels = 0;
+  if (!done) enqueue_multipoles(h);
   int max_nf = nf0;
   while (!done && nf > 0) {
     const int nb = div_up(nf, kTravBlock);
