@@ -404,7 +404,8 @@ __global__ void __launch_bounds__(256)
 }
 
 // Steps of a tile are split into kLatSeg segments; segment s runs on XCD s
-// (workgroup w = 8 tile + s, and workgroups are dealt round-robin over the
+// (work item 8 tile + s is taken by workgroups w = item mod gridDim.x, the
+// grid is a multiple of 8, and workgroups are dealt round-robin over the
 // 8 XCDs), so an XCD only ever needs the T(d) of one eighth of the offsets
 // and keeps them in its L2.
 constexpr int kLatSeg = 8;
@@ -779,7 +780,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
     // the items (room left on every CU for the P2P waves of the other stream).
     const int nitems = ntiles * kLatSeg;
     int ngrid = nitems;
-    if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, h.lat_wg_cu * h.ncu);
+    if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, std::max(kLatSeg, (h.lat_wg_cu * h.ncu) / kLatSeg * kLatSeg));
     const dim3 grid(ngrid);
     EXA_HIP(hipEventRecord(h.ev[12], h.stream));
     h.lat_timed = true;

@@ -1,7 +1,7 @@
 // P2P over the CSR interaction list: a 64-lane workgroup works on one target
 // leaf at a time, taking the next leaf from a work counter when done (the
-// grid is one wave per leaf, or smaller so that the lattice M2L kernel of the
-// other stream fits beside it on every CU).
+// grid is one wave per leaf, or a resident grid of a few waves per CU so that
+// the lattice M2L kernels of the other stream fit beside it on every CU).
 //
 // Reference: kernels/LaplaceP2PCPU.cxx:141-169 (pair) and :172-331 (self).
 // Per pair: dX = x_i - x_j - X_periodic, R2 = |dX|^2, skipped when R2 == 0;
