@@ -93,6 +93,8 @@ static void do_upward(Handle &h) {
 // Lists on the main stream; then P2P on the second stream concurrently with
 // M2L on the main stream (independent outputs: body results vs local
 // expansions); the main stream waits for P2P before anything downstream.
+// The multipoles of a split upward pass (third stream) are joined by the
+// M2L paths right before they read them.
 static void do_traverse(Handle &h) {
   ensure(h.have_up, "traverse: upward pass not run");
   EXA_HIP(hipEventRecord(h.ev[4], h.stream));
@@ -104,11 +106,8 @@ static void do_traverse(Handle &h) {
   run_p2p(h, h.stream2);
   EXA_HIP(hipEventRecord(h.ev[10], h.stream2));
   EXA_HIP(hipEventRecord(h.ev[6], h.stream));
-  if (h.mult_pending) {  // multipoles from the split upward pass
-    EXA_HIP(hipStreamWaitEvent(h.stream, h.ev_mult, 0));
-    h.mult_pending = false;
-  }
   run_m2l(h);
+  join_multipoles(h);  // if no M2L path did
   if (h.cfg.images > 0) run_periodic(h);
   EXA_HIP(hipEventRecord(h.ev[11], h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream, h.ev[10], 0));
@@ -179,10 +178,12 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     EXA_HIP(hipDeviceGetAttribute(&H->h.ncu, hipDeviceAttributeMultiprocessorCount, H->h.cfg.device));
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream, hipStreamNonBlocking));
     EXA_HIP(hipStreamCreateWithFlags(&H->h.stream2, hipStreamNonBlocking));
+    EXA_HIP(hipStreamCreateWithFlags(&H->h.stream3, hipStreamNonBlocking));
     for (auto &e : H->h.ev) EXA_HIP(hipEventCreate(&e));
     EXA_HIP(hipEventCreate(&H->h.ev_mult));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_cnt, hipEventDisableTiming));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_lists, hipEventDisableTiming));
+    EXA_HIP(hipEventCreateWithFlags(&H->h.ev_p2pcnt, hipEventDisableTiming));
     EXA_HIP(hipHostMalloc((void **)&H->h.hpin, 64 * sizeof(int), hipHostMallocDefault));
   } catch (const Error &e) {
     delete H;
@@ -513,6 +514,7 @@ int exafmm_hip_count_p2p_bodies(exafmm_hip_handle *H, int64_t *n) {
 
 int exafmm_hip_synchronize(exafmm_hip_handle *H) {
   return guarded(H, [&](Handle &h) {
+    EXA_HIP(hipStreamSynchronize(h.stream3));
     EXA_HIP(hipStreamSynchronize(h.stream2));
     EXA_HIP(hipStreamSynchronize(h.stream));
   });

@@ -73,12 +73,14 @@ struct Handle {
   std::string err;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // P2P runs here, concurrently with M2L
+  hipStream_t stream3 = nullptr;  // multipoles of the split upward pass, beside the lists
   hipEvent_t ev[18] = {};
-  hipEvent_t ev_mult = nullptr;  // multipoles done on stream2 (split upward pass)
+  hipEvent_t ev_mult = nullptr;  // multipoles done on stream3 (split upward pass)
   bool mult_pending = false;     // the main stream has not joined ev_mult yet
   bool mult_enqueue = false;     // run_upward_mult not enqueued yet (the traversal does it)
   hipEvent_t ev_cnt = nullptr;   // a device count copied into hpin
   hipEvent_t ev_lists = nullptr; // raw lists done (the P2P grouping on stream2 waits for it)
+  hipEvent_t ev_p2pcnt = nullptr; // P2P target count copied into hpin[9] (stream2)
   int *hpin = nullptr;           // pinned host words for counts read back behind queued work
 
   // Bodies.
@@ -114,7 +116,8 @@ struct Handle {
   DBuf<unsigned char> p2p_img, m2l_img;
   DBuf<int> p2p_tgt, m2l_tgt;      // compact target lists
   DBuf<int> p2p_next;              // P2P work counter (next target leaf)
-  int np2p_tgt = 0, nm2l_tgt = 0;
+  int np2p_tgt = 0, nm2l_tgt = 0;  // np2p_tgt < 0: on the device only (p2p_cnt_dev)
+  const int *p2p_cnt_dev = nullptr;
   DBuf<int> raw_mi, raw_mj, raw_pi, raw_pj;  // emission-order lists
   DBuf<unsigned char> raw_mimg, raw_pimg;
 
@@ -209,14 +212,28 @@ void run_build_forest(Handle &h);
 void run_build_adaptive(Handle &h);
 void run_upward(Handle &h);
 void run_upward_radii(Handle &h);  // split upward pass: radii on the main stream
-void run_upward_mult(Handle &h);   // ... multipoles on the second stream after them
+void run_upward_mult(Handle &h);   // ... multipoles on the third stream after them
 // run_upward_mult once, if the split upward pass left it to be enqueued.
 inline void enqueue_multipoles(Handle &h) {
   if (!h.mult_enqueue) return;
   h.mult_enqueue = false;
   run_upward_mult(h);
 }
+// The main stream joins the multipoles of the split upward pass (before it
+// reads M or the lattice's scaled multipoles).
+inline void join_multipoles(Handle &h) {
+  if (!h.mult_pending) return;
+  EXA_HIP(hipStreamWaitEvent(h.stream, h.ev_mult, 0));
+  h.mult_pending = false;
+}
 void run_lists(Handle &h);
+// The P2P target count on the host (run_lists leaves it on the device, so
+// that nothing waits for the P2P grouping before P2P and M2L are queued).
+inline void resolve_p2p_count(Handle &h) {
+  if (h.np2p_tgt >= 0) return;
+  EXA_HIP(hipEventSynchronize(h.ev_p2pcnt));
+  h.np2p_tgt = h.hpin[9];
+}
 // The M2L list grouped by target (run_lists leaves it raw for the lattice M2L).
 void ensure_m2l_grouped(Handle &h);
 // Pairs (tgt, src, img) grouped by target into CSR (off, src_out, img_out)

@@ -322,7 +322,10 @@ static void m2l_impl(Handle &h, const M2LList &lst) {
 
 void run_m2l(Handle &h) {
   if (h.nm2l == 0) return;
-  if (h.cfg.basis == 1) return run_m2l_cart(h);
+  if (h.cfg.basis == 1) {
+    join_multipoles(h);
+    return run_m2l_cart(h);
+  }
   // Lattice pairs on the matrix cores first (from the raw list; lst becomes
   // the irregular pairs grouped by target); otherwise every pair per pair.
   h.pair_timed = false;
@@ -331,6 +334,7 @@ void run_m2l(Handle &h) {
     ensure_m2l_grouped(h);
     lst = M2LList{h.m2l_tgt.p, h.nm2l_tgt, h.m2l_off.p, h.m2l_src.p, h.m2l_img.p};
   }
+  join_multipoles(h);  // (the lattice path joined them before its contraction)
   switch (h.cfg.P) {
     case 4: m2l_impl<4>(h, lst); break;
     case 6: m2l_impl<6>(h, lst); break;

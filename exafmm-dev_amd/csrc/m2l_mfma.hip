@@ -774,6 +774,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
   h.lat_stats_tiles = ntiles;
   // 6. The contraction (timed by ev[12..13]) and the reduction.
   const bool run_mfma = noff > 0 && ntiles > 0 && nsteps_all > 0;
+  join_multipoles(h);  // the set-up above did not need them
   if (run_mfma) {
     h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
     // One workgroup per item, or lat_wg_cu workgroups per CU looping over

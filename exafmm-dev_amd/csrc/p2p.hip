@@ -148,8 +148,9 @@ struct TargetPasses {
 };
 
 __global__ void __launch_bounds__(64)
-    k_p2p_f64(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
-              const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+    k_p2p_f64(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
+              int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
+              const unsigned char *__restrict__ img, const int *__restrict__ cBody,
               const int *__restrict__ cNBody, const double4 *__restrict__ body, double4 *__restrict__ trg,
               double cycle) {
   __shared__ double4 sb[kP2PStage];
@@ -158,6 +159,8 @@ __global__ void __launch_bounds__(64)
   const int lane = threadIdx.x;
   // Target leaves: blockIdx.x first, then the next unclaimed one of the
   // work counter (leaves differ in cost; the grid may be smaller than ntl).
+  // ntl_dev (if set): the count as the list grouping left it on the device.
+  const int ntl = ntl_dev ? *ntl_dev : ntl_host;
   for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
@@ -298,8 +301,9 @@ struct Kahan32 {
 };
 
 __global__ void __launch_bounds__(64)
-    k_p2p_mixed(const int *__restrict__ tgt_list, int ntl, int *__restrict__ next, const int *__restrict__ off,
-                const int *__restrict__ src, const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+    k_p2p_mixed(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
+                int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
+                const unsigned char *__restrict__ img, const int *__restrict__ cBody,
                 const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
                 double4 *__restrict__ trg, double cycle) {
   __shared__ float4 sb[kP2PStageF];   // hi parts and charge
@@ -309,6 +313,8 @@ __global__ void __launch_bounds__(64)
   const int lane = threadIdx.x;
   // Target leaves: blockIdx.x first, then the next unclaimed one of the
   // work counter (leaves differ in cost; the grid may be smaller than ntl).
+  // ntl_dev (if set): the count as the list grouping left it on the device.
+  const int ntl = ntl_dev ? *ntl_dev : ntl_host;
   for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
@@ -468,20 +474,30 @@ void run_p2p(Handle &h, hipStream_t st) {
   // CU to the other stream's kernels (the lattice M2L set-up runs beside P2P
   // instead of queueing behind one wave per leaf); 12 waves (10.6 KB of LDS
   // and at most 128 VGPRs each) leave a third of the LDS and of the VGPRs.
-  int grid = h.np2p_tgt;
-  if (h.p2p_wpc > 0) grid = std::min(grid, h.p2p_wpc * h.ncu);
+  // The target count is read on the device when the host has not read it
+  // back yet and the grid does not depend on it (a resident grid).
+  const int *ntl_dev = nullptr;
+  int grid = h.p2p_wpc * h.ncu;
+  if (h.np2p_tgt < 0 && h.p2p_wpc > 0) {
+    ntl_dev = h.p2p_cnt_dev;
+  } else {
+    resolve_p2p_count(h);
+    if (h.np2p_tgt == 0) return;
+    grid = h.p2p_wpc > 0 ? std::min(h.np2p_tgt, grid) : h.np2p_tgt;
+  }
   h.p2p_next.reserve(1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, h.p2p_next.p);
   if (h.cfg.precision == 1)
-    hipLaunchKernelGGL(k_p2p_mixed, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, h.p2p_next.p, h.p2p_off.p,
-                       h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+    hipLaunchKernelGGL(k_p2p_mixed, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   else
-    hipLaunchKernelGGL(k_p2p_f64, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, h.p2p_next.p, h.p2p_off.p,
-                       h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+    hipLaunchKernelGGL(k_p2p_f64, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   EXA_HIP(hipGetLastError());
 }
 
 int64_t count_p2p_body_pairs(Handle &h) {
+  resolve_p2p_count(h);
   if (h.np2p_tgt == 0) return 0;
   h.red.reserve(2);
   EXA_HIP(hipMemsetAsync(h.red.p, 0, sizeof(unsigned long long), h.stream));

@@ -334,6 +334,7 @@ static void downward_impl(Handle &h) {
       const int pc = h.lvl_off[t] - h.lvl_off[t - 1];
       if (pc > 0 && h.lvl_off[t + 1] > h.lvl_off[t]) run_l2l_level_mfma(h, h.lvl_cells.p + h.lvl_off[t - 1], pc);
     }
+    resolve_p2p_count(h);
     if (h.np2p_tgt > 0)
       hipLaunchKernelGGL((k_downward<P, true>), dim3(div_up(h.np2p_tgt, 2)), dim3(64), 0, h.stream, h.p2p_tgt.p,
                          h.body4.p, h.cX.p, h.cParent.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.L.p, h.trg4.p,
@@ -381,10 +382,10 @@ void run_upward(Handle &h) {
   EXA_DISPATCH_P3(upward_impl, h, 3, h.stream);
 }
 
-// The upward pass split over the two streams: radii (needed by the
-// traversal) on the main stream, multipoles (needed only by M2L) on the
-// second stream after the radii, joined by the caller through h.ev_mult
-// before M is read.  run_upward_mult is enqueued by the traversal once its
+// The upward pass split over two streams: radii (needed by the traversal)
+// on the main stream, multipoles (needed only by M2L) on the third stream
+// after the radii, joined by the main stream through h.ev_mult before M is
+// read (join_multipoles).  run_upward_mult is enqueued by the traversal once its
 // own first kernels are queued (its launches would otherwise hold the host
 // while the main stream idles).
 void run_upward_radii(Handle &h) {
@@ -396,11 +397,11 @@ void run_upward_radii(Handle &h) {
 }
 
 void run_upward_mult(Handle &h) {
-  EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[16], 0));
-  EXA_HIP(hipEventRecord(h.ev[17], h.stream2));
-  EXA_DISPATCH_P3(upward_impl, h, 1, h.stream2);
-  run_lattice_mscale(h, h.stream2);  // M2L input, ahead of P2P on this stream
-  EXA_HIP(hipEventRecord(h.ev_mult, h.stream2));
+  EXA_HIP(hipStreamWaitEvent(h.stream3, h.ev[16], 0));
+  EXA_HIP(hipEventRecord(h.ev[17], h.stream3));
+  EXA_DISPATCH_P3(upward_impl, h, 1, h.stream3);
+  run_lattice_mscale(h, h.stream3);  // the lattice M2L's input
+  EXA_HIP(hipEventRecord(h.ev_mult, h.stream3));
 }
 
 void run_downward(Handle &h) {

@@ -493,11 +493,12 @@ struct GroupCtx {
 // Group (tgt[k], src[k], img[k]) by target: CSR offsets off[0..ncells], the
 // sources and images in target order (stable w.r.t. emission order), and the
 // compact list of targets with at least one entry, whose count is copied to
-// *ntl_host (pinned; valid after g.st is synchronised).  Nothing waits for
-// the device.
+// *ntl_host (pinned; valid after g.st is synchronised) and left on the
+// device at *ntl_dev (if given; valid until the scratch g.e is reused).
+// Nothing waits for the device.
 static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, DBuf<int> &src,
                             DBuf<unsigned char> &img, DBuf<int> &off, DBuf<int> &src_out, DBuf<unsigned char> &img_out,
-                            DBuf<int> &tlist, int *ntl_host) {
+                            DBuf<int> &tlist, int *ntl_host, const int **ntl_dev = nullptr) {
   const int nc = h.ncells;
   const int T = 256;
   off.resize(nc + 1);
@@ -530,6 +531,7 @@ static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, D
   tlist.resize(nc);
   hipLaunchKernelGGL(k_compact, dim3(div_up(nc, T)), dim3(T), 0, g.st, flag, scan, nc, tlist.p);
   EXA_HIP(hipMemcpyAsync(ntl_host, scan + nc, sizeof(int), hipMemcpyDeviceToHost, g.st));
+  if (ntl_dev) *ntl_dev = scan + nc;
 }
 
 // Levels of device-driven passes, enqueued without waiting; returns false
@@ -657,6 +659,9 @@ void run_lists(Handle &h) {
   // list is grouped by target only for the other M2L paths (and on request,
   // `ensure_m2l_grouped`).  With both grouped, the two groupings run at once:
   // M2L on the main stream, P2P on the second stream (its own scratch).
+  // The P2P grouping's target count stays on the device (a resident P2P
+  // grid reads it there; resolve_p2p_count reads it back when the host needs
+  // it), so the host queues P2P and the M2L set-up without waiting for it.
   h.m2l_grouped = !lattice_applies(h);
   EXA_HIP(hipEventRecord(h.ev_lists, h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev_lists, 0));
@@ -664,11 +669,14 @@ void run_lists(Handle &h) {
     group_by_target(h, GroupCtx{h.ia, h.ib, h.ic, h.id, h.ie, h.tmp, h.stream}, nm, mi, mj, mimg, h.m2l_off,
                     h.m2l_src, h.m2l_img, h.m2l_tgt, h.hpin + 8);
   group_by_target(h, GroupCtx{h.g2a, h.g2b, h.g2c, h.g2d, h.g2e, h.tmp2, h.stream2}, np, pi, pj, pimg, h.p2p_off,
-                  h.p2p_src, h.p2p_img, h.p2p_tgt, h.hpin + 9);
-  EXA_HIP(hipStreamSynchronize(h.stream2));
-  EXA_HIP(hipStreamSynchronize(h.stream));
-  h.nm2l_tgt = h.m2l_grouped ? h.hpin[8] : 0;
-  h.np2p_tgt = h.hpin[9];
+                  h.p2p_src, h.p2p_img, h.p2p_tgt, h.hpin + 9, &h.p2p_cnt_dev);
+  EXA_HIP(hipEventRecord(h.ev_p2pcnt, h.stream2));
+  h.np2p_tgt = -1;
+  h.nm2l_tgt = 0;
+  if (h.m2l_grouped) {
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    h.nm2l_tgt = h.hpin[8];
+  }
   EXA_HIP(hipGetLastError());
 }
 

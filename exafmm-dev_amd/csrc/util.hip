@@ -62,9 +62,11 @@ Handle::~Handle() {
   if (ev_mult) (void)hipEventDestroy(ev_mult);
   if (ev_cnt) (void)hipEventDestroy(ev_cnt);
   if (ev_lists) (void)hipEventDestroy(ev_lists);
+  if (ev_p2pcnt) (void)hipEventDestroy(ev_p2pcnt);
   if (hpin) (void)hipHostFree(hpin);
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
+  if (stream3) (void)hipStreamDestroy(stream3);
 }
 
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n) { exclusive_scan_int(in, out, n, h.stream, h.tmp); }
