@@ -2,6 +2,7 @@
 // handle destructor.  Sorting and scanning are bookkeeping around the hot
 // path (tree build and list grouping); the hot kernels are hand-written.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "internal.h"
 
@@ -92,31 +93,52 @@ void exclusive_scan_int4(Handle &h, const int4 *in, int4 *out, int n) {
   EXA_HIP(hipcub::DeviceScan::ExclusiveScan(h.tmp.p, bytes, in, out, Int4Sum{}, zero, n, h.stream));
 }
 
+// Stable radix sorts of (key, int) pairs over key bits [0, bits).  rocprim's
+// default configuration sorts up to 2^20 items by a block sort and merge
+// passes (about 20 launches for the 2^20 body keys of a 1M-body tree); from
+// kOnesweepMin items on, the configuration here takes the onesweep radix sort
+// (one pass per 8 key bits).  Smaller sorts keep the default: they may run
+// beside P2P, where onesweep's chained blocks wait for CUs (a 37K-item sort
+// took 2 ms there instead of 0.17).
+using OnesweepConfig =
+    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, rocprim::default_config, 0>;
+constexpr int kOnesweepMin = 1 << 18;
+
+template <typename Config, typename K>
+static void radix_sort_pairs_cfg(K *keys, K *keys_alt, int *vals, int *vals_alt, int n, int bits,
+                                 bool &result_in_alt, hipStream_t st, DBuf<unsigned char> &tmp) {
+  rocprim::double_buffer<K> k(keys, keys_alt);
+  rocprim::double_buffer<int> v(vals, vals_alt);
+  const unsigned int end_bit = bits > 0 ? bits : 1;
+  size_t bytes = 0;
+  EXA_HIP(rocprim::radix_sort_pairs<Config>(nullptr, bytes, k, v, n, 0u, end_bit, st));
+  tmp.reserve(bytes);
+  EXA_HIP(rocprim::radix_sort_pairs<Config>(tmp.p, bytes, k, v, n, 0u, end_bit, st));
+  result_in_alt = k.current() != keys;
+}
+
+template <typename K>
+static void radix_sort_pairs(K *keys, K *keys_alt, int *vals, int *vals_alt, int n, int bits, bool &result_in_alt,
+                             hipStream_t st, DBuf<unsigned char> &tmp) {
+  if (n >= kOnesweepMin)
+    radix_sort_pairs_cfg<OnesweepConfig>(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, st, tmp);
+  else
+    radix_sort_pairs_cfg<rocprim::default_config>(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, st, tmp);
+}
+
 void sort_pairs_u64_int(Handle &h, uint64_t *keys, uint64_t *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt) {
-  hipcub::DoubleBuffer<uint64_t> k(keys, keys_alt);
-  hipcub::DoubleBuffer<int> v(vals, vals_alt);
-  size_t bytes = 0;
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
-  h.tmp.reserve(bytes);
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(h.tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, h.stream));
-  result_in_alt = k.Current() != keys;
+  radix_sort_pairs(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, h.stream, h.tmp);
 }
 
 void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits,
                         bool &result_in_alt) {
-  sort_pairs_int_int(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, h.stream, h.tmp);
+  radix_sort_pairs(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, h.stream, h.tmp);
 }
 
 void sort_pairs_int_int(int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits, bool &result_in_alt,
                         hipStream_t st, DBuf<unsigned char> &tmp) {
-  hipcub::DoubleBuffer<int> k(keys, keys_alt);
-  hipcub::DoubleBuffer<int> v(vals, vals_alt);
-  size_t bytes = 0;
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, n, 0, bits > 0 ? bits : 1, st));
-  tmp.reserve(bytes);
-  EXA_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, k, v, n, 0, bits > 0 ? bits : 1, st));
-  result_in_alt = k.Current() != keys;
+  radix_sort_pairs(keys, keys_alt, vals, vals_alt, n, bits, result_in_alt, st, tmp);
 }
 
 }  // namespace exafmm
