@@ -33,7 +33,12 @@ __device__ inline void image_shift(int img, double cycle, double &px, double &py
 }
 
 constexpr int kP2PStage = 256;  // source bodies staged in LDS per fill
-constexpr int kP2PUnroll = 4;   // FP64 pairs in flight per lane
+// FP64 pairs in flight per lane, side by side: written one pair after the
+// other, the compiler ran each pair's dependent chain (LDS read, wait, 18
+// dependent operations) alone; two side by side measured fastest (four need
+// 156 VGPRs, and three such waves per SIMD leave no room for the kernels of
+// the other streams).
+constexpr int kP2PUnroll = 2;
 static_assert(kP2PStage % (8 * kP2PUnroll) == 0, "stages are padded to multiples of kP2PUnroll G, G <= 8");
 constexpr double kFar = 1e100;  // position of the zero-charge padding bodies
 
@@ -169,18 +174,6 @@ __global__ void __launch_bounds__(64)
     // leaf, or a periodic image), so R2 == 0 is skipped by a select; other
     // stages have R2 > 0 for every pair (equal positions share a Morton key
     // and hence a leaf).
-    auto pair = [](auto mask, const double4 &xi, const double4 &xj, double4 &acc) {
-      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double rs = rsqrt_pos(r2);
-      const double invr = decltype(mask)::value ? (r2 > 0 ? rs : 0.0) : rs;
-      const double qinvr = xj.w * invr;
-      const double qinvr3 = qinvr * invr * invr;
-      acc.x += qinvr;
-      acc.y += dx * qinvr3;
-      acc.z += dy * qinvr3;
-      acc.w += dz * qinvr3;
-    };
     // Sum the G group partials of the pass's targets in group order.
     auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
       __syncthreads();
@@ -243,8 +236,31 @@ __global__ void __launch_bounds__(64)
           constexpr int G = decltype(gc)::value;
           const double4 *sg = sb + g;
           for (int j = 0; j < ncp; j += kP2PUnroll * G) {
+            // The kP2PUnroll pairs side by side (loads, then each step of
+            // the pair arithmetic for all of them), summed in source order.
+            double4 xj[kP2PUnroll];
+            double dx[kP2PUnroll], dy[kP2PUnroll], dz[kP2PUnroll], r2[kP2PUnroll], rs[kP2PUnroll];
 #pragma unroll
-            for (int u = 0; u < kP2PUnroll; u++) pair(mask, x, sg[j + u * G], acc);
+            for (int u = 0; u < kP2PUnroll; u++) xj[u] = sg[j + u * G];
+#pragma unroll
+            for (int u = 0; u < kP2PUnroll; u++) {
+              dx[u] = x.x - xj[u].x;
+              dy[u] = x.y - xj[u].y;
+              dz[u] = x.z - xj[u].z;
+              r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            }
+#pragma unroll
+            for (int u = 0; u < kP2PUnroll; u++) rs[u] = rsqrt_pos(r2[u]);
+#pragma unroll
+            for (int u = 0; u < kP2PUnroll; u++) {
+              const double invr = decltype(mask)::value ? (r2[u] > 0 ? rs[u] : 0.0) : rs[u];
+              const double qinvr = xj[u].w * invr;
+              const double qinvr3 = qinvr * invr * invr;
+              acc.x += qinvr;
+              acc.y += dx[u] * qinvr3;
+              acc.z += dy[u] * qinvr3;
+              acc.w += dz[u] * qinvr3;
+            }
           }
         };
         auto pass = [&](auto mask, int G, const double4 &x, int g,
@@ -287,6 +303,7 @@ __global__ void __launch_bounds__(64)
 constexpr int kP2PStageF = 256;
 static_assert(kP2PStageF % 64 == 0, "stage tiles are padded to multiples of 64");
 constexpr int kSeg = 8;  // FP32 pair terms summed before each compensated add
+constexpr int kSegRun = 2;  // of which side by side (as kP2PUnroll in k_p2p_f64)
 constexpr float kFarF = 1e18f;  // padding position: R2 = 3e36 stays finite in FP32
 
 struct Kahan32 {
@@ -347,21 +364,36 @@ __global__ void __launch_bounds__(64)
       constexpr int G = decltype(gc)::value;
       const float4 *sg = sb + k0, *sgl = sbl + k0;
       float pot = 0, ax = 0, ay = 0, az = 0;
+      // The kSeg pairs in runs of kSegRun side by side (loads, then each
+      // step of the pair arithmetic for all of them), summed in source order.
 #pragma unroll
-      for (int u = 0; u < kSeg; u++) {
-        const float4 xj = sg[u * G];
-        const float4 xl = sgl[u * G];
-        const float dx = (t.x - xj.x) + (t.xl - xl.x), dy = (t.y - xj.y) + (t.yl - xl.y),
-                    dz = (t.z - xj.z) + (t.zl - xl.z);
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float rs = __builtin_amdgcn_rsqf(r2);
-        const float invr = decltype(mask)::value ? (r2 > 0.0f ? rs : 0.0f) : rs;
-        const float qinvr = xj.w * invr;
-        const float qinvr3 = qinvr * invr * invr;
-        pot += qinvr;
-        ax += dx * qinvr3;
-        ay += dy * qinvr3;
-        az += dz * qinvr3;
+      for (int u0 = 0; u0 < kSeg; u0 += kSegRun) {
+        float4 xj[kSegRun], xl[kSegRun];
+        float dx[kSegRun], dy[kSegRun], dz[kSegRun], r2[kSegRun], rs[kSegRun];
+#pragma unroll
+        for (int u = 0; u < kSegRun; u++) {
+          xj[u] = sg[(u0 + u) * G];
+          xl[u] = sgl[(u0 + u) * G];
+        }
+#pragma unroll
+        for (int u = 0; u < kSegRun; u++) {
+          dx[u] = (t.x - xj[u].x) + (t.xl - xl[u].x);
+          dy[u] = (t.y - xj[u].y) + (t.yl - xl[u].y);
+          dz[u] = (t.z - xj[u].z) + (t.zl - xl[u].z);
+          r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kSegRun; u++) rs[u] = __builtin_amdgcn_rsqf(r2[u]);
+#pragma unroll
+        for (int u = 0; u < kSegRun; u++) {
+          const float invr = decltype(mask)::value ? (r2[u] > 0.0f ? rs[u] : 0.0f) : rs[u];
+          const float qinvr = xj[u].w * invr;
+          const float qinvr3 = qinvr * invr * invr;
+          pot += qinvr;
+          ax += dx[u] * qinvr3;
+          ay += dy[u] * qinvr3;
+          az += dz[u] * qinvr3;
+        }
       }
       a.p.add(pot);
       a.x.add(ax);
