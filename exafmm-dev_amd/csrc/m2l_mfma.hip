@@ -550,18 +550,30 @@ __global__ void __launch_bounds__(256)
   constexpr DegreeTable<P> deg{};
   const int tile = blockIdx.x;
   double *Lr = reinterpret_cast<double *>(L);
-  // Rows 16 blockIdx.y .. 16 blockIdx.y + 15 of the tile.
-  const int r0 = 16 * blockIdx.y, r1 = min(NR, r0 + 16);
-  for (int i = r0 * 64 + threadIdx.x; i < r1 * 64; i += 256) {
-    const int row = i / 64, col = i % 64;
-    const int tgt = tile_tgt[tile * 64 + col];
+  // Rows r0 .. r0 + 15 of the tile: the segment sums read column-fastest
+  // (as the partials lie), then written row-fastest through LDS (the 16 rows
+  // of one target are 128 contiguous bytes of L).
+  __shared__ double sv[16][65];
+  const int r0 = 16 * blockIdx.y, nrow = min(NR, r0 + 16) - r0;
+  const int col = threadIdx.x & 63, rq = threadIdx.x >> 6;
+  double v[4] = {0, 0, 0, 0};
+  const double *pp = part + ((size_t)tile * kLatSeg * NRP + r0) * 64 + col;
+  for (int sg = 0; sg < kLatSeg; sg++)
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (rq + 4 * k < nrow) v[k] += pp[((size_t)sg * NRP + rq + 4 * k) * 64];
+#pragma unroll
+  for (int k = 0; k < 4; k++) sv[rq + 4 * k][col] = v[k];
+  __syncthreads();
+  const int row = threadIdx.x & 15;
+  if (row >= nrow) return;
+  for (int c = threadIdx.x >> 4; c < 64; c += 16) {
+    const int tgt = tile_tgt[tile * 64 + c];
     if (tgt < 0) continue;
-    double v = 0;
-    for (int sg = 0; sg < kLatSeg; sg++) v += part[((size_t)(tile * kLatSeg + sg) * NRP + row) * 64 + col];
     const double iw = 1.0 / (2 * cRb[tgt] / kLatUnit);
     double f = iw;
-    for (int q = 0; q < deg.n[row >> 1]; q++) f *= iw;
-    Lr[(size_t)tgt * NR + row] += v * f;
+    for (int q = 0; q < deg.n[(r0 + row) >> 1]; q++) f *= iw;
+    Lr[(size_t)tgt * NR + r0 + row] += sv[row][c] * f;
   }
 }
 

@@ -287,6 +287,64 @@ __global__ void __launch_bounds__(512)
     }
 }
 
+// L2L of a level with few parents: one workgroup per (16 parents, row
+// block), so a level of a few cells still spreads over RB workgroups instead
+// of running its whole operator product in one (the small levels took about
+// 30 us each, bound by one workgroup's operator loads and MFMA chain).
+template <int P, int R>
+__device__ inline void l2l_row_block(const double *__restrict__ Op, int lane, const double (&b)[UDShape<P>::KC],
+                                     v4d_ud &acc) {
+  using U = UDShape<P>;
+  constexpr int KC = U::KC, K0 = U::l2l_kbeg(R);
+  double a[KC - K0];
+#pragma unroll
+  for (int kc = K0; kc < KC; kc++) a[kc - K0] = Op[(R * KC + kc) * 64 + lane];
+#pragma unroll
+  for (int kc = K0; kc < KC; kc++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc - K0], b[kc], acc, 0, 0, 0);
+}
+
+template <int P, int R = 0>
+__device__ inline void l2l_row_block_of(int rb, const double *__restrict__ Op, int lane,
+                                        const double (&b)[UDShape<P>::KC], v4d_ud &acc) {
+  if constexpr (R < UDShape<P>::RB) {
+    if (rb == R) return l2l_row_block<P, R>(Op, lane, b, acc);
+    l2l_row_block_of<P, R + 1>(rb, Op, lane, b, acc);
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(512)
+    k_l2l_mfma_rb(const int *__restrict__ cells, int ncell, const double4 *__restrict__ cX,
+                  const double *__restrict__ cRb, const int *__restrict__ cChild, const int *__restrict__ cNChild,
+                  const double *__restrict__ B, cplx *__restrict__ L) {
+  using U = UDShape<P>;
+  constexpr int NR = U::NR, KC = U::KC;
+  constexpr DegTab<P> deg{};
+  __shared__ UDCols<P> s;
+  ud_setup<P>(s, cells, ncell, cX, cRb, cChild, cNChild);
+  const int tid = threadIdx.x, o = tid >> 6, lane = tid & 63;  // wave o: octant o
+  const int col = lane & 15, kq = lane >> 4, rb = blockIdx.y;
+  double *Lr = reinterpret_cast<double *>(L);
+  const int p = s.cell[col];
+  const double *src = Lr + (size_t)(p < 0 ? 0 : p) * NR;
+  double b[KC];
+#pragma unroll
+  for (int kc = 0; kc < KC; kc++) {
+    const int k = 4 * kc + kq;
+    const double v = src[k < NR ? k : NR - 1] * s.pw[col][deg.d[k]];
+    b[kc] = (p >= 0 && k < NR) ? v : 0.0;
+  }
+  v4d_ud acc = v4d_ud{0, 0, 0, 0};
+  l2l_row_block_of<P>(rb, B + (size_t)o * U::RB * KC * 64, lane, b, acc);
+  const int c = s.ch[col][o];
+  if (c < 0 || p < 0) return;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int row = rb * 16 + (lane >> 4) + 4 * r;
+    if (row < NR) Lr[(size_t)c * NR + row] += s.zero[col] ? src[row] : acc[r] * s.ipw[col][deg.d[row]];
+  }
+}
+
 __global__ void k_fill_f64(double *__restrict__ a, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -318,8 +376,15 @@ template <int P>
 static void l2l_level(Handle &h, const int *cells, int cnt) {
   using U = UDShape<P>;
   ensure_ops<P>(h, h.stream);
-  hipLaunchKernelGGL(k_l2l_mfma<P>, dim3(div_up(cnt, 16)), dim3(512), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
-                     h.cChild.p, h.cNChild.p, h.ud_ops.p + U::OPS, h.L.p);
+  // Levels of fewer than kL2LSplit groups of 16 parents: one workgroup per
+  // (group, row block).
+  constexpr int kL2LSplit = 64;
+  if (div_up(cnt, 16) < kL2LSplit)
+    hipLaunchKernelGGL(k_l2l_mfma_rb<P>, dim3(div_up(cnt, 16), U::RB), dim3(512), 0, h.stream, cells, cnt, h.cX.p,
+                       h.cRb.p, h.cChild.p, h.cNChild.p, h.ud_ops.p + U::OPS, h.L.p);
+  else
+    hipLaunchKernelGGL(k_l2l_mfma<P>, dim3(div_up(cnt, 16)), dim3(512), 0, h.stream, cells, cnt, h.cX.p, h.cRb.p,
+                       h.cChild.p, h.cNChild.p, h.ud_ops.p + U::OPS, h.L.p);
 }
 
 // Periodic runs keep the per-cell kernels: the image sums cancel strongly
