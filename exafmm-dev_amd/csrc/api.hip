@@ -102,10 +102,13 @@ static void do_traverse(Handle &h) {
   enqueue_multipoles(h);  // if the traversal did not
   EXA_HIP(hipEventRecord(h.ev[5], h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[5], 0));
+  // The M2L set-up's first kernels are queued before P2P: they run while the
+  // P2P list is grouped, before P2P fills the GPU.
+  EXA_HIP(hipEventRecord(h.ev[6], h.stream));
+  m2l_begin(h);
   EXA_HIP(hipEventRecord(h.ev[9], h.stream2));
   run_p2p(h, h.stream2);
   EXA_HIP(hipEventRecord(h.ev[10], h.stream2));
-  EXA_HIP(hipEventRecord(h.ev[6], h.stream));
   run_m2l(h);
   join_multipoles(h);  // if no M2L path did
   if (h.cfg.images > 0) run_periodic(h);
@@ -184,7 +187,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_cnt, hipEventDisableTiming));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_lists, hipEventDisableTiming));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_p2pcnt, hipEventDisableTiming));
-    EXA_HIP(hipHostMalloc((void **)&H->h.hpin, 64 * sizeof(int), hipHostMallocDefault));
+    EXA_HIP(hipHostMalloc((void **)&H->h.hpin, Handle::kPinWords * sizeof(int), hipHostMallocDefault));
   } catch (const Error &e) {
     delete H;
     return e.code;

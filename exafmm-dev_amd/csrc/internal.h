@@ -82,6 +82,7 @@ struct Handle {
   hipEvent_t ev_lists = nullptr; // raw lists done (the P2P grouping on stream2 waits for it)
   hipEvent_t ev_p2pcnt = nullptr; // P2P target count copied into hpin[9] (stream2)
   int *hpin = nullptr;           // pinned host words for counts read back behind queued work
+  static constexpr int kPinWords = 512;  // hpin: [0..64) counts, [64..322) lattice tile histogram, ...
 
   // Bodies.
   int n = 0;
@@ -162,6 +163,8 @@ struct Handle {
   int lat_tc_rows = 0, lat_tc_P = 0, lat_tc_max = 16384;  // rows; a fuller table starts afresh
   bool lat_tc_f32 = false;
   bool lat_ms_ready = false;     // lat_Ms / lat_Msf already queued for this evaluation
+  bool lat_begun = false;        // lattice set-up's first part queued (m2l_begin)
+  bool lat_tile_alt = false;     // sorted tile keys in the second half of lat_tkey / lat_tval
   DBuf<double> lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;
   int64_t lat_stats_pairs = 0;
@@ -227,6 +230,9 @@ inline void join_multipoles(Handle &h) {
   h.mult_pending = false;
 }
 void run_lists(Handle &h);
+// The first part of the lattice M2L set-up (nothing waits for the device),
+// queued on the main stream before P2P; run_m2l does the rest.
+void m2l_begin(Handle &h);
 // The P2P target count on the host (run_lists leaves it on the device, so
 // that nothing waits for the P2P grouping before P2P and M2L are queued).
 inline void resolve_p2p_count(Handle &h) {

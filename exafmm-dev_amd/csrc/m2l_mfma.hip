@@ -409,6 +409,9 @@ __global__ void __launch_bounds__(256)
 // 8 XCDs), so an XCD only ever needs the T(d) of one eighth of the offsets
 // and keeps them in its L2.
 constexpr int kLatSeg = 8;
+// Words of Handle::hpin the lattice set-up reads back.
+constexpr int kPinGhist = 64, kPinNoff = 64 + 256, kPinNnew = 64 + 257;
+static_assert(kPinNnew < Handle::kPinWords, "pinned words");
 
 typedef __attribute__((address_space(3))) void lds_void;
 // s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left alone.
@@ -627,8 +630,7 @@ static void grow_keep_rows(Handle &h, DBuf<R> &b, size_t want, size_t keep) {
 }
 
 template <int P>
-static void lattice_run(Handle &h, M2LList &irr) {
-  using S = MfmaShape<P>;
+static void lattice_begin(Handle &h) {
   const int nc = h.ncells;
   const int64_t np = h.nm2l;
   const int TPB = 256;
@@ -672,14 +674,41 @@ static void lattice_run(Handle &h, M2LList &irr) {
                      tk, tv);
   bool alt = false;
   sort_pairs_int_int(h, tk, tk + nc, tv, tv + nc, nc, 8, alt);
-  const int *skey = alt ? tk + nc : tk, *sval = alt ? tv + nc : tv;
+  h.lat_tile_alt = alt;
+  const int *skey = alt ? tk + nc : tk;
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, h.stream, h.lat_thist.p, (int64_t)256, 0);
   hipLaunchKernelGGL(k_tile_hist, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, skey, h.lat_thist.p);
-  int ghist[256], noff = 0, nnew = 0;
-  EXA_HIP(hipMemcpyAsync(ghist, h.lat_thist.p, sizeof(ghist), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipMemcpyAsync(&noff, h.lat_kscan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipMemcpyAsync(&nnew, h.lat_tc_scan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  // Tile histogram, offset count and new-row count for the host (read by
+  // lattice_run after the main stream has passed these copies).
+  EXA_HIP(hipMemcpyAsync(h.hpin + kPinGhist, h.lat_thist.p, 256 * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(h.hpin + kPinNoff, h.lat_kscan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipMemcpyAsync(h.hpin + kPinNnew, h.lat_tc_scan.p + kLatKeys, sizeof(int), hipMemcpyDeviceToHost,
+                         h.stream));
+  h.lat_begun = true;
+}
+
+// The rest of the lattice M2L, after lattice_begin (which the evaluation
+// queues before P2P, so the set-up's first kernels start while the P2P
+// list is grouped, before P2P fills the GPU).
+template <int P>
+static void lattice_run(Handle &h, M2LList &irr) {
+  using S = MfmaShape<P>;
+  const int nc = h.ncells;
+  const int64_t np = h.nm2l;
+  const int TPB = 256;
+  const int gp = div_up(np, TPB);
+  const bool f32 = h.cfg.precision == 1;
+  if (!h.lat_begun) lattice_begin<P>(h);
+  h.lat_begun = false;
+  const int *ptgt = h.raw_mi.p, *psrc = h.raw_mj.p;
+  const unsigned char *pimg = h.raw_mimg.p;
+  const int *tk = h.lat_tkey.p, *tv = h.lat_tval.p;
+  const int *skey = h.lat_tile_alt ? tk + nc : tk, *sval = h.lat_tile_alt ? tv + nc : tv;
   EXA_HIP(hipStreamSynchronize(h.stream));
+  int ghist[256];
+  std::copy(h.hpin + kPinGhist, h.hpin + kPinGhist + 256, ghist);
+  const int noff = h.hpin[kPinNoff];
+  int nnew = h.hpin[kPinNnew];
   if (h.lat_tc_rows + nnew > h.lat_tc_max) {  // start the table afresh
     tc_reset(h, P, f32);
     hipLaunchKernelGGL(k_lat_flag, dim3(div_up(kLatKeys + 1, TPB)), dim3(TPB), 0, h.stream, h.lat_hist.p,
@@ -847,6 +876,19 @@ void run_lattice_mscale(Handle &h, hipStream_t st) {
   }
   EXA_HIP(hipGetLastError());
   h.lat_ms_ready = true;
+}
+
+void m2l_begin(Handle &h) {
+  h.lat_begun = false;
+  if (!lattice_applies(h) || h.nm2l == 0) return;
+  switch (h.cfg.P) {
+    case 4: lattice_begin<4>(h); break;
+    case 6: lattice_begin<6>(h); break;
+    case 8: lattice_begin<8>(h); break;
+    case 10: lattice_begin<10>(h); break;
+    case 12: lattice_begin<12>(h); break;
+  }
+  EXA_HIP(hipGetLastError());
 }
 
 bool run_m2l_lattice(Handle &h, M2LList &irr) {
