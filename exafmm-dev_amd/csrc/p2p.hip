@@ -159,7 +159,10 @@ __global__ void __launch_bounds__(64)
               const int *__restrict__ cNBody, const double4 *__restrict__ body, double4 *__restrict__ trg,
               double cycle) {
   __shared__ double4 sb[kP2PStage];
-  __shared__ double4 sacc[64];
+  // The group partials are combined in the stage buffer (every stage is
+  // consumed by then): 8.6 KB of LDS per wave, which leaves more of each CU
+  // to the other streams' kernels beside twelve resident waves.
+  double4 *const sacc = sb;
   __shared__ SrcDesc sd;
   const int lane = threadIdx.x;
   // Target leaves: blockIdx.x first, then the next unclaimed one of the
@@ -323,9 +326,9 @@ __global__ void __launch_bounds__(64)
                 const unsigned char *__restrict__ img, const int *__restrict__ cBody,
                 const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
                 double4 *__restrict__ trg, double cycle) {
-  __shared__ float4 sb[kP2PStageF];   // hi parts and charge
-  __shared__ float4 sbl[kP2PStageF];  // lo parts
-  __shared__ double4 sacc[64];
+  __shared__ __attribute__((aligned(32))) float4 sb[kP2PStageF];  // hi parts and charge
+  __shared__ float4 sbl[kP2PStageF];                               // lo parts
+  double4 *const sacc = reinterpret_cast<double4 *>(sb);           // (as in k_p2p_f64)
   __shared__ SrcDesc sd;
   const int lane = threadIdx.x;
   // Target leaves: blockIdx.x first, then the next unclaimed one of the
