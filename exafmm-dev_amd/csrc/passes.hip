@@ -165,6 +165,43 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Radii alone (what = 2 of k_upward: setRmax, up_down_pass.h:51-67, and
+// R /= theta, :82), one wave per cell and four cells per workgroup, without
+// k_upward's P2M tables in LDS (which held it to two waves per CU: the leaf
+// level took 85 us).  The maximum is exact in any order.
+__global__ void __launch_bounds__(256)
+    k_radii(const int *__restrict__ cells, int ncell, const double4 *__restrict__ body, double4 *__restrict__ cX,
+            const double *__restrict__ cRb, const int *__restrict__ cChild, const int *__restrict__ cNChild,
+            const int *__restrict__ cBody, const int *__restrict__ cNBody, double theta, int useRmax) {
+  const int w = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= ncell) return;  // wave-uniform
+  const int c = cells[w];
+  const double4 Xc = cX[c];
+  const int nchild = cNChild[c];
+  double rmax = 0;
+  if (nchild == 0) {
+    const int b0 = cBody[c], nb = cNBody[c];
+    for (int i = lane; i < nb; i += 64) {
+      const double4 v = body[b0 + i];
+      const double r = exact_dist(Xc.x, Xc.y, Xc.z, v.x, v.y, v.z);
+      rmax = r > rmax ? r : rmax;
+    }
+  } else if (lane < nchild) {
+    const double4 Xj = cX[cChild[c] + lane];
+    rmax = exact_dist(Xc.x, Xc.y, Xc.z, Xj.x, Xj.y, Xj.z) + Xj.w;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double o = __shfl_xor(rmax, d);
+    rmax = o > rmax ? o : rmax;
+  }
+  if (lane == 0) {
+    double R = cRb[c];
+    if (useRmax) R = R < rmax ? R : rmax;
+    R /= theta;
+    cX[c].w = R;
+  }
+}
+
 // L2L (LaplaceSphericalCPU.cxx:193-224) for every cell of one level, then
 // L2P (:226-256) for the leaves among them.
 //
@@ -314,6 +351,11 @@ static void upward_impl(Handle &h, int what, hipStream_t st) {
     const int cnt = h.lvl_off[t + 1] - h.lvl_off[t];
     if (cnt == 0) continue;
     const int *cells = h.lvl_cells.p + h.lvl_off[t];
+    if (what == 2) {
+      hipLaunchKernelGGL(k_radii, dim3(div_up(cnt, 4)), dim3(256), 0, st, cells, cnt, h.body4.p, h.cX.p, h.cRb.p,
+                         h.cChild.p, h.cNChild.p, h.cBody.p, h.cNBody.p, h.cfg.theta, h.cfg.useRmax);
+      continue;
+    }
     hipLaunchKernelGGL(k_upward<P>, dim3(cnt), dim3(64), 0, st, cells, h.body4.p, h.cX.p, h.cRb.p, h.cChild.p,
                        h.cNChild.p, h.cBody.p, h.cNBody.p, h.M.p, h.cfg.theta, h.cfg.useRmax, mf ? 0 : 1, what);
     if ((what & 1) && mf && t + 1 < h.nlevels) run_m2m_level_mfma(h, cells, cnt, st);
