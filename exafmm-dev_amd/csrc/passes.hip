@@ -313,6 +313,9 @@ __global__ void __launch_bounds__(64)
     };
 #pragma unroll
     for (int m = 0; m < P; m++) {
+      // One order m at a time: without the barrier the scheduler issued the
+      // LDS reads of every term up front (256 VGPRs, one wave per SIMD).
+      __builtin_amdgcn_sched_barrier(0);
       double p = pn;
       double p1 = p;
       p = x * (2 * m + 1) * p1;
