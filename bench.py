@@ -294,8 +294,8 @@ def main():
         {"batched": bench_batched, "plummer": bench_plummer, "periodic": bench_periodic}[args.config](args)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X; multi-GPU sharding is design-only "
-              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the evaluation is not sharded over several GPUs. Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     n, P = args.n, args.P
