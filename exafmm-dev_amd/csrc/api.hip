@@ -12,10 +12,6 @@
 using exafmm::Error;
 using exafmm::Handle;
 
-struct exafmm_hip_handle {
-  Handle h;
-};
-
 namespace exafmm {
 
 // SetRopt (include/up_down_pass.h:15-42,125-129).  powf/std::pow of the host
@@ -68,8 +64,6 @@ static void ensure(bool cond, const char *msg) {
   if (!cond) throw Error(EXAFMM_HIP_EINVAL, msg);
 }
 
-static void set_device(Handle &h) { EXA_HIP(hipSetDevice(h.cfg.device)); }
-
 static void do_build(Handle &h) {
   ensure(h.have_bounds, "build_tree: bounds not computed");
   ensure(h.n > 0, "build_tree: no bodies");
@@ -81,10 +75,12 @@ static void do_build(Handle &h) {
     run_build_adaptive(h);
   h.have_tree = true;
   h.have_up = h.have_trav = false;
+  h.lat_ms_ready = false;  // scaled multipoles of an older tree
 }
 
 static void do_upward(Handle &h) {
   ensure(h.have_tree, "upward_pass: tree not built");
+  h.lat_ms_ready = false;  // the lattice M2L rescales these multipoles itself
   run_upward(h);
   run_ropt(h);
   h.have_up = true;
@@ -128,23 +124,6 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }  // namespace exafmm
 
 using namespace exafmm;
-
-template <typename F>
-static int guarded(exafmm_hip_handle *H, F &&f) {
-  if (!H) return EXAFMM_HIP_EINVAL;
-  try {
-    set_device(H->h);
-    f(H->h);
-    H->h.err.clear();
-    return EXAFMM_HIP_OK;
-  } catch (const Error &e) {
-    H->h.err = e.what();
-    return e.code;
-  } catch (const std::exception &e) {
-    H->h.err = e.what();
-    return EXAFMM_HIP_EDEVICE;
-  }
-}
 
 extern "C" {
 
@@ -212,10 +191,21 @@ int exafmm_hip_set_config(exafmm_hip_handle *H, const exafmm_hip_config *cfg) {
     ensure(cfg->P == h.cfg.P && cfg->device == h.cfg.device && cfg->basis == h.cfg.basis,
            "set_config: P, basis and device are fixed at creation");
     if (int rc = check_config(cfg)) throw Error(rc, "set_config: invalid configuration");
-    const bool tree_changed = cfg->ncrit != h.cfg.ncrit || cfg->builder != h.cfg.builder;
-    h.cfg = *cfg;
-    if (h.cfg.nspawn <= 0) h.cfg.nspawn = 1000;
-    if (tree_changed) h.have_tree = h.have_up = h.have_trav = false;
+    exafmm_hip_config c = *cfg;
+    if (c.nspawn <= 0) c.nspawn = 1000;
+    const exafmm_hip_config &o = h.cfg;
+    // What each pass's output depends on: the tree (ncrit, builder), the
+    // upward pass (cell radii: theta, useRmax; SetRopt; MASS normalisation)
+    // and the traversal (lists: nspawn, mutual, images, cycle; arithmetic:
+    // precision).  A change invalidates that pass and everything after it.
+    const bool tree_changed = c.ncrit != o.ncrit || c.builder != o.builder;
+    const bool up_changed = c.theta != o.theta || c.useRmax != o.useRmax || c.useRopt != o.useRopt || c.mass != o.mass;
+    const bool trav_changed = c.nspawn != o.nspawn || c.mutual != o.mutual || c.images != o.images ||
+                              c.cycle != o.cycle || c.precision != o.precision;
+    h.cfg = c;
+    if (tree_changed) h.have_tree = false;
+    if (tree_changed || up_changed) h.have_up = false;
+    if (tree_changed || up_changed || trav_changed) h.have_trav = false;
   });
 }
 

@@ -461,10 +461,10 @@ __global__ void __launch_bounds__(64)
 }
 
 template <int P, int IPL>
-static void m2l_cart_launch(Handle &h, const uint32_t *desc, const int *slot0, const int *outslot, int maxc) {
-  hipLaunchKernelGGL((k_m2l_cart<P, IPL>), dim3(h.nm2l_tgt), dim3(64), 0, h.stream, h.m2l_tgt.p, h.m2l_off.p,
-                     h.m2l_src.p, h.m2l_img.p, h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, maxc, h.cfg.cycle,
-                     h.cfg.mass);
+static void m2l_cart_launch(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L,
+                            const uint32_t *desc, const int *slot0, const int *outslot, int maxc) {
+  hipLaunchKernelGGL((k_m2l_cart<P, IPL>), dim3(lst.ntgt), dim3(64), 0, h.stream, lst.tgt, lst.off, lst.src, lst.img,
+                     cX, M, L, desc, slot0, outslot, maxc, h.cfg.cycle, h.cfg.mass);
 }
 
 // Items per lane, rounded up to the compiled register budgets.
@@ -484,7 +484,7 @@ struct CartIPL<8> {
 };
 
 template <int P>
-static void m2l_cart_impl(Handle &h) {
+static void m2l_cart_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L) {
   using S = CartM2LShape<P>;
   const int key = 1000 + P;
   if (h.m2l_sched_P != key) {
@@ -505,8 +505,14 @@ static void m2l_cart_impl(Handle &h) {
   const uint32_t *desc = h.m2l_sched.p;
   const int *slot0 = reinterpret_cast<const int *>(desc + 64 * CartIPL<P>::V);
   const int *outslot = slot0 + 64;
-  m2l_cart_launch<P, CartIPL<P>::V>(h, desc, slot0, outslot, h.cart_maxc);
+  if (lst.ntgt > 0) m2l_cart_launch<P, CartIPL<P>::V>(h, lst, cX, M, L, desc, slot0, outslot, h.cart_maxc);
   EXA_HIP(hipGetLastError());
+}
+
+template <int P>
+static void m2l_cart_impl(Handle &h) {
+  m2l_cart_arrays<P>(h, M2LList{h.m2l_tgt.p, h.nm2l_tgt, h.m2l_off.p, h.m2l_src.p, h.m2l_img.p}, h.cX.p, h.M.p,
+                     h.L.p);
 }
 
 // ---------------------------------------------------------------------------
@@ -686,6 +692,44 @@ void run_m2l_cart(Handle &h) {
 void run_periodic_cart(Handle &h) {
   if (h.cfg.images <= 0 || h.ncells == 0) return;
   EXA_DISPATCH_CART(periodic_cart_impl, h);
+}
+
+// The Cartesian pass kernels on a caller-built cell set (per-cell operators).
+template <int P>
+static void upward_cart_cells_impl(Handle &h, const CellSetView &v) {
+  hipLaunchKernelGGL(k_upward_cart<P>, dim3(v.ncell), dim3(64), 0, h.stream, v.cells, v.body, v.cX, v.cRb, v.cChild,
+                     v.cNChild, v.cBody, v.cNBody, v.M, h.cfg.theta, h.cfg.useRmax);
+}
+
+template <int P>
+static void downward_cart_cells_impl(Handle &h, const CellSetView &v) {
+  hipLaunchKernelGGL(k_downward_cart<P>, dim3(v.ncell), dim3(64), 0, h.stream, v.cells, v.body, v.cX, v.cParent,
+                     v.cNChild, v.cBody, v.cNBody, v.M, v.L, v.trg, v.do_l2l, h.cfg.mass);
+}
+
+#define EXA_DISPATCH_CART2(fn, h, a)                                              \
+  switch ((h).cfg.P) {                                                            \
+    case 4: fn<4>(h, a); break;                                                   \
+    case 6: fn<6>(h, a); break;                                                   \
+    case 8: fn<8>(h, a); break;                                                   \
+    default: throw Error(EXAFMM_HIP_EUNSUP, "Cartesian P must be one of 4, 6, 8"); \
+  }
+
+void launch_upward_cart_cells(Handle &h, const CellSetView &v) {
+  EXA_DISPATCH_CART2(upward_cart_cells_impl, h, v);
+  EXA_HIP(hipGetLastError());
+}
+void launch_downward_cart_cells(Handle &h, const CellSetView &v) {
+  EXA_DISPATCH_CART2(downward_cart_cells_impl, h, v);
+  EXA_HIP(hipGetLastError());
+}
+void launch_m2l_cart_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L) {
+  switch (h.cfg.P) {
+    case 4: m2l_cart_arrays<4>(h, lst, cX, M, L); break;
+    case 6: m2l_cart_arrays<6>(h, lst, cX, M, L); break;
+    case 8: m2l_cart_arrays<8>(h, lst, cX, M, L); break;
+    default: throw Error(EXAFMM_HIP_EUNSUP, "Cartesian P must be one of 4, 6, 8");
+  }
 }
 
 }  // namespace exafmm

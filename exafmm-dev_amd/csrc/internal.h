@@ -191,6 +191,14 @@ struct Handle {
   int trav_fcap = 0, trav_levels = 0;
   DBuf<int> cnt;                   // small device counters
   DBuf<double> red;                // reduction scratch
+  // Per-cell operators (cell_ops.hip): their small cell set, bodies and
+  // results, apart from the handle's tree.
+  DBuf<double4> co_X, co_body, co_trg, co_src;
+  DBuf<double4> co_trg2;
+  DBuf<double> co_Rb;
+  DBuf<cplx> co_M, co_L;
+  DBuf<int> co_int;
+  DBuf<unsigned char> co_img;
 
   exafmm_hip_stats stats = {};
 
@@ -206,6 +214,21 @@ struct M2LList {
   const int *off;
   const int *src;
   const unsigned char *img;
+};
+
+// Device arrays of a small cell set that the per-cell operators of the
+// C-ABI build and hand to the pass kernels: the cells to process (`cells`,
+// ncell entries), the cell columns, expansions, bodies and results.
+struct CellSetView {
+  const int *cells = nullptr;
+  int ncell = 0;
+  const double4 *body = nullptr;
+  double4 *cX = nullptr;
+  const double *cRb = nullptr;
+  const int *cParent = nullptr, *cChild = nullptr, *cNChild = nullptr, *cBody = nullptr, *cNBody = nullptr;
+  cplx *M = nullptr, *L = nullptr;
+  double4 *trg = nullptr;
+  int do_l2l = 0;
 };
 
 // Phase implementations (one translation unit each).
@@ -270,6 +293,16 @@ void run_ewald(Handle &h, int ksize, double alpha, double sigma, double cutoff, 
 void run_periodic(Handle &h);
 int64_t count_p2p_body_pairs(Handle &h);
 
+// Pass kernels on a caller-built cell set (per-cell operators).
+void launch_upward_cells(Handle &h, const CellSetView &v);
+void launch_downward_cells(Handle &h, const CellSetView &v);
+void launch_upward_cart_cells(Handle &h, const CellSetView &v);
+void launch_downward_cart_cells(Handle &h, const CellSetView &v);
+// M2L of the pairs of `lst` on explicit centre and expansion arrays in FP64
+// (spherical per-pair kernel, or the Cartesian kernel for basis 1).
+void launch_m2l_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L);
+void launch_m2l_cart_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L);
+
 // Scan / sort helpers (hipCUB) in util.hip.
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n);
 // The same on stream st with temporary storage tmp (concurrent users).
@@ -282,4 +315,30 @@ void sort_pairs_int_int(Handle &h, int *keys, int *keys_alt, int *vals, int *val
 void sort_pairs_int_int(int *keys, int *keys_alt, int *vals, int *vals_alt, int n, int bits, bool &result_in_alt,
                         hipStream_t st, DBuf<unsigned char> &tmp);
 
+inline void set_device(Handle &h) { EXA_HIP(hipSetDevice(h.cfg.device)); }
+
 }  // namespace exafmm
+
+// The opaque handle of the C-ABI.
+struct exafmm_hip_handle {
+  exafmm::Handle h;
+};
+
+// Runs f on the handle's device and turns exceptions into status codes,
+// keeping the message on the handle (every C-ABI entry point goes through it).
+template <typename F>
+static int guarded(exafmm_hip_handle *H, F &&f) {
+  if (!H) return EXAFMM_HIP_EINVAL;
+  try {
+    exafmm::set_device(H->h);
+    f(H->h);
+    H->h.err.clear();
+    return EXAFMM_HIP_OK;
+  } catch (const exafmm::Error &e) {
+    H->h.err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    H->h.err = e.what();
+    return EXAFMM_HIP_EDEVICE;
+  }
+}

@@ -289,9 +289,10 @@ __global__ void __launch_bounds__(64)
 }
 
 template <int P>
-static void m2l_impl(Handle &h, const M2LList &lst) {
+static void m2l_launch(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L, bool f32,
+                       bool timed) {
   using S = M2LShape<P>;
-  const int esz = h.cfg.precision == 1 ? 8 : 16;  // bytes of one table entry
+  const int esz = f32 ? 8 : 16;  // bytes of one table entry
   const int key = P * 100 + esz;
   if (h.m2l_sched_P != key) {
     const M2LSchedule sc = build_schedule(P, S::IPL, esz);
@@ -308,20 +309,45 @@ static void m2l_impl(Handle &h, const M2LList &lst) {
   const int *slot0 = reinterpret_cast<const int *>(desc + 64 * S::IPL);
   const int *outslot = slot0 + 64;
   if (lst.ntgt == 0) return;
-  EXA_HIP(hipEventRecord(h.ev[14], h.stream));
-  h.pair_timed = true;
+  if (timed) {
+    EXA_HIP(hipEventRecord(h.ev[14], h.stream));
+    h.pair_timed = true;
+  }
   if (esz == 8)
     hipLaunchKernelGGL((k_m2l<P, float>), dim3(lst.ntgt), dim3(64), 0, h.stream, lst.tgt, lst.off, lst.src, lst.img,
-                       h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+                       cX, M, L, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
   else
     hipLaunchKernelGGL((k_m2l<P, double>), dim3(lst.ntgt), dim3(64), 0, h.stream, lst.tgt, lst.off, lst.src, lst.img,
-                       h.cX.p, h.M.p, h.L.p, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
-  EXA_HIP(hipEventRecord(h.ev[15], h.stream));
+                       cX, M, L, desc, slot0, outslot, h.cfg.cycle, h.cfg.mass);
+  if (timed) EXA_HIP(hipEventRecord(h.ev[15], h.stream));
   EXA_HIP(hipGetLastError());
 }
 
+template <int P>
+static void m2l_impl(Handle &h, const M2LList &lst) {
+  m2l_launch<P>(h, lst, h.cX.p, h.M.p, h.L.p, h.cfg.precision == 1, true);
+}
+
+void launch_m2l_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L) {
+  if (h.cfg.basis == 1) return launch_m2l_cart_arrays(h, lst, cX, M, L);
+  switch (h.cfg.P) {
+    case 4: m2l_launch<4>(h, lst, cX, M, L, false, false); break;
+    case 6: m2l_launch<6>(h, lst, cX, M, L, false, false); break;
+    case 8: m2l_launch<8>(h, lst, cX, M, L, false, false); break;
+    case 10: m2l_launch<10>(h, lst, cX, M, L, false, false); break;
+    case 12: m2l_launch<12>(h, lst, cX, M, L, false, false); break;
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
+  }
+}
+
 void run_m2l(Handle &h) {
-  if (h.nm2l == 0) return;
+  if (h.nm2l == 0) {
+    // No pair reads the scaled multipoles the split upward pass may have
+    // queued: a later traversal must rebuild them for its own tree.
+    h.lat_ms_ready = false;
+    h.lat_begun = false;
+    return;
+  }
   if (h.cfg.basis == 1) {
     join_multipoles(h);
     return run_m2l_cart(h);

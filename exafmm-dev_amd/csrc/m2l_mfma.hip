@@ -880,7 +880,10 @@ void run_lattice_mscale(Handle &h, hipStream_t st) {
 
 void m2l_begin(Handle &h) {
   h.lat_begun = false;
-  if (!lattice_applies(h) || h.nm2l == 0) return;
+  if (!lattice_applies(h) || h.nm2l == 0) {
+    h.lat_ms_ready = false;  // queued scaled multipoles (if any) go unused
+    return;
+  }
   switch (h.cfg.P) {
     case 4: lattice_begin<4>(h); break;
     case 6: lattice_begin<6>(h); break;

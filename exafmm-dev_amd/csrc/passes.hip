@@ -454,4 +454,45 @@ void run_downward(Handle &h) {
   EXA_DISPATCH_P(downward_impl, h);
 }
 
+// The pass kernels on a small cell array built by the caller (the per-cell
+// operators of the C-ABI): multipoles (P2M at leaves, M2M per cell from its
+// children) of `cells`, and L2L (do_l2l) then L2P at leaves of `cells`.
+template <int P>
+static void upward_cells_impl(Handle &h, const CellSetView &v) {
+  hipLaunchKernelGGL(k_upward<P>, dim3(v.ncell), dim3(64), 0, h.stream, v.cells, v.body, v.cX, v.cRb, v.cChild,
+                     v.cNChild, v.cBody, v.cNBody, v.M, h.cfg.theta, h.cfg.useRmax, 1, 1);
+}
+
+template <int P>
+static void downward_cells_impl(Handle &h, const CellSetView &v) {
+  hipLaunchKernelGGL((k_downward<P, false>), dim3(v.ncell), dim3(64), 0, h.stream, v.cells, v.body, v.cX,
+                     v.cParent, v.cNChild, v.cBody, v.cNBody, v.M, v.L, v.trg, v.do_l2l, h.cfg.mass, v.ncell);
+}
+
+void launch_upward_cells(Handle &h, const CellSetView &v) {
+  if (h.cfg.basis == 1) return launch_upward_cart_cells(h, v);
+  switch (h.cfg.P) {
+    case 4: upward_cells_impl<4>(h, v); break;
+    case 6: upward_cells_impl<6>(h, v); break;
+    case 8: upward_cells_impl<8>(h, v); break;
+    case 10: upward_cells_impl<10>(h, v); break;
+    case 12: upward_cells_impl<12>(h, v); break;
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_downward_cells(Handle &h, const CellSetView &v) {
+  if (h.cfg.basis == 1) return launch_downward_cart_cells(h, v);
+  switch (h.cfg.P) {
+    case 4: downward_cells_impl<4>(h, v); break;
+    case 6: downward_cells_impl<6>(h, v); break;
+    case 8: downward_cells_impl<8>(h, v); break;
+    case 10: downward_cells_impl<10>(h, v); break;
+    case 12: downward_cells_impl<12>(h, v); break;
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
+  }
+  EXA_HIP(hipGetLastError());
+}
+
 }  // namespace exafmm

@@ -25,6 +25,8 @@ EXPORTS = (
     "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
     "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald", "exafmm_hip_count_p2p_bodies",
     "exafmm_hip_evaluate_batch",
+    "exafmm_hip_cell_p2p", "exafmm_hip_cell_p2p_self", "exafmm_hip_cell_p2m", "exafmm_hip_cell_m2m",
+    "exafmm_hip_cell_m2l", "exafmm_hip_cell_l2l", "exafmm_hip_cell_l2p",
 )
 
 
@@ -95,6 +97,13 @@ def load():
     L.exafmm_hip_get_lists.argtypes = [vp, vp, vp, vp, vp]
     L.exafmm_hip_direct.argtypes = [vp, ip, vp, vp]
     L.exafmm_hip_direct_arrays.argtypes = [vp, ip, vp, ip, vp, vp]
+    L.exafmm_hip_cell_p2p.argtypes = [vp, ip, vp, vp, ip, vp, vp, dp, vp, ip]
+    L.exafmm_hip_cell_p2p_self.argtypes = [vp, ip, vp, vp, dp]
+    L.exafmm_hip_cell_p2m.argtypes = [vp, ip, vp, vp, vp]
+    L.exafmm_hip_cell_m2m.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.exafmm_hip_cell_m2l.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ip]
+    L.exafmm_hip_cell_l2l.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.exafmm_hip_cell_l2p.argtypes = [vp, ip, vp, vp, vp, vp]
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]

@@ -69,8 +69,8 @@ class BuildTree {
       C.WEIGHT = 0;
       for (int d = 0; d < 3; d++) C.X[d] = X[3 * c + d];
       C.R = R[c];
-      C.M = complex_t(0);
-      C.L = complex_t(0);
+      C.M = 0;
+      C.L = 0;
     }
     exafmm_hip_stats s;
     ctx.check(exafmm_hip_get_stats(ctx.h, &s), "exafmm_hip_get_stats");

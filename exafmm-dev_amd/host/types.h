@@ -14,9 +14,6 @@
 #ifndef EXPANSION
 #error "define EXPANSION (the expansion order P), as the reference build does"
 #endif
-#ifdef Cartesian
-#error "the GPU path implements the spherical basis (-DSpherical)"
-#endif
 
 typedef double real_t;
 const real_t EPS = 1e-16;
@@ -26,8 +23,32 @@ typedef vec<4, real_t> kvec4;
 typedef real_t kreal_t;
 
 const int P = EXPANSION;
+#if Cartesian
+// kernels/LaplaceCartesianCPU.cxx: real Taylor coefficients (types.h:40-42).
+const int NTERM = P * (P + 1) * (P + 2) / 6;
+typedef vec<NTERM, real_t> vecP;
+#else
 const int NTERM = P * (P + 1) / 2;
 typedef vec<NTERM, complex_t> vecP;
+#endif
+
+// An expansion in the C-ABI's layout (2 * NTERM doubles, re/im pairs; the
+// Cartesian basis keeps its real coefficients in the real parts).
+inline void expansion_to_abi(const vecP &v, double *out) {
+  for (int t = 0; t < NTERM; t++) {
+    out[2 * t] = std::real(v[t]);
+    out[2 * t + 1] = std::imag(v[t]);
+  }
+}
+inline void expansion_from_abi(const double *in, vecP &v) {
+  for (int t = 0; t < NTERM; t++) {
+#if Cartesian
+    v[t] = in[2 * t];
+#else
+    v[t] = complex_t(in[2 * t], in[2 * t + 1]);
+#endif
+  }
+}
 
 struct Box {
   vec3 X;

@@ -73,15 +73,13 @@ class UpDownPass {
   // Copy the device expansions into Cell::M and Cell::L (host inspection).
   void fetchExpansions(Cells &cells) {
     exafmm_host::Context &ctx = exafmm_host::context();
-    std::vector<complex_t> M(cells.size() * NTERM), L(cells.size() * NTERM);
-    ctx.check(exafmm_hip_get_cells(ctx.h, nullptr, nullptr, nullptr, nullptr, reinterpret_cast<double *>(M.data()),
-                                   reinterpret_cast<double *>(L.data())),
+    std::vector<double> M(cells.size() * 2 * NTERM), L(cells.size() * 2 * NTERM);
+    ctx.check(exafmm_hip_get_cells(ctx.h, nullptr, nullptr, nullptr, nullptr, M.data(), L.data()),
               "exafmm_hip_get_cells");
-    for (size_t c = 0; c < cells.size(); c++)
-      for (int t = 0; t < NTERM; t++) {
-        cells[c].M[t] = M[c * NTERM + t];
-        cells[c].L[t] = L[c * NTERM + t];
-      }
+    for (size_t c = 0; c < cells.size(); c++) {
+      expansion_from_abi(&M[c * 2 * NTERM], cells[c].M);
+      expansion_from_abi(&L[c * 2 * NTERM], cells[c].L);
+    }
   }
 };
 #endif

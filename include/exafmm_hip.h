@@ -8,9 +8,9 @@
  *
  * Each entry point names the reference interface it replaces (paths are
  * relative to the reference repository root).  The C++ classes in
- * exafmm-dev_amd/host/ (BoundBox, BuildTree, UpDownPass, Traversal, kernel::)
- * forward to these calls so that an examples/serial.cxx-shaped driver runs on
- * the GPU unchanged.
+ * exafmm-dev_amd/host/ (BoundBox, BuildTree, UpDownPass, Traversal, and the
+ * per-cell kernel:: functions) forward to these calls so that an
+ * examples/serial.cxx-shaped driver runs on the GPU unchanged.
  */
 #ifndef EXAFMM_HIP_H
 #define EXAFMM_HIP_H
@@ -37,8 +37,10 @@ typedef struct {
   int nspawn;     /* split-both threshold of the traversal (args.h: -s)        */
   int useRmax;    /* args.h: -x                                                */
   int useRopt;    /* args.h: -o                                                */
-  int mutual;     /* args.h: -m; accepted, evaluated as the equivalent
-                     non-mutual interaction set (target-owned writes)          */
+  int mutual;     /* args.h: -m; 1 reproduces the pair set of the mutual
+                     recursion (traversal.h:108-120,199), recorded in both
+                     directions; evaluation stays target-owned (no mutual
+                     scatter)                                                  */
   int images;     /* periodic image sublevels (args.h: -i)                     */
   double cycle;   /* periodic cycle (examples/serial.cxx:12)                   */
   int builder;    /* 0: uniform Morton tree (include/build_tree_omp.h),
@@ -77,9 +79,12 @@ typedef struct {
 int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out);
 int exafmm_hip_destroy(exafmm_hip_handle *h);
 /* Change the run-time parameters between passes (theta, ncrit, nspawn,
- * useRmax, useRopt, mutual, images, cycle, builder, mass, precision); P and
- * device are fixed at creation.  The reference reads them from the pass
- * objects' constructors (examples/serial.cxx:15-22). */
+ * useRmax, useRopt, mutual, images, cycle, builder, mass, precision); P,
+ * basis and device are fixed at creation.  A changed value invalidates the
+ * pass that depends on it and every later one: ncrit or builder the tree;
+ * theta, useRmax, useRopt or mass the upward pass; nspawn, mutual, images,
+ * cycle or precision the traversal -- rerun from there.  The reference reads
+ * them from the pass objects' constructors (examples/serial.cxx:15-22). */
 int exafmm_hip_set_config(exafmm_hip_handle *h, const exafmm_hip_config *cfg);
 const char *exafmm_hip_last_error(const exafmm_hip_handle *h);
 
@@ -179,6 +184,53 @@ int exafmm_hip_count_p2p_bodies(exafmm_hip_handle *h, int64_t *n);
 
 /* Wait for all work queued on the handle's stream. */
 int exafmm_hip_synchronize(exafmm_hip_handle *h);
+
+/* ---- Per-cell operators: the reference's kernel API ---------------------
+ * namespace kernel { P2P, P2P, P2M, M2M, M2L, L2L, L2P } of
+ * include/kernel.h:6-14 on caller-held cells, evaluated on the device in
+ * FP64 in the handle's basis and MASS mode, independent of the handle's own
+ * tree.  A cell is its centre X (3 doubles) and, where used, its expansions
+ * (2*NTERM doubles, re/im pairs in the reference's coefficient order; the
+ * Cartesian basis uses the real parts, im = 0); bodies are n x (x, y, z, q)
+ * and results n x 4 in the reference's Body::TRG convention: P2P adds the
+ * unnormalised q_i q_j / r and q_i q_j grad terms, L2P divides TRG by the
+ * charge before it adds the local field (kernels/LaplaceSphericalCPU.cxx:
+ * 226-256, kernels/LaplaceP2PCPU.cxx:141-331).  Every operator adds into
+ * its outputs, as the reference's do.  Each call is one small launch with
+ * its own copies: the batched passes above are the fast path.
+ * exafmm-dev_amd/host/kernel.h binds the kernel:: functions to these. */
+
+/* kernel::P2P(Ci, Cj, eps2, Xperiodic, mutual), kernels/LaplaceP2PCPU.cxx:
+ * 141-169: targets ti (ni) against sources sj (nj) shifted by Xperiodic
+ * (NULL = 0); trg_i gets the target sums; with mutual, trg_j the reaction
+ * terms.  ti and sj are distinct cells (the self pair is p2p_self). */
+int exafmm_hip_cell_p2p(exafmm_hip_handle *h, int ni, const double *ti, double *trg_i, int nj, const double *sj,
+                        double *trg_j, double eps2, const double *Xperiodic, int mutual);
+/* kernel::P2P(C, eps2), kernels/LaplaceP2PCPU.cxx:172-331: every pair of
+ * the cell once, both bodies updated. */
+int exafmm_hip_cell_p2p_self(exafmm_hip_handle *h, int n, const double *xyzq, double *trg, double eps2);
+/* kernel::P2M(C), kernels/LaplaceSphericalCPU.cxx:103-118: M += the
+ * multipole of the n bodies about X. */
+int exafmm_hip_cell_p2m(exafmm_hip_handle *h, int n, const double *xyzq, const double *X, double *M);
+/* kernel::M2M(Ci, C0), kernels/LaplaceSphericalCPU.cxx:120-147: M += the
+ * multipoles Mc of nchild children (centres Xc, 3 per child) shifted to X. */
+int exafmm_hip_cell_m2m(exafmm_hip_handle *h, int nchild, const double *Xc, const double *Mc, const double *X,
+                        double *M);
+/* kernel::M2L(Ci, Cj, Xperiodic, mutual), kernels/LaplaceSphericalCPU.cxx:
+ * 149-191: Li += the translation of Mj (source centre Xj + Xperiodic,
+ * Xperiodic NULL = 0); with mutual, Lj += that of Mi.  Mi is read for
+ * mutual and in MASS mode (M0 products), Lj is written only with mutual. */
+int exafmm_hip_cell_m2l(exafmm_hip_handle *h, const double *Xi, const double *Mi, double *Li, const double *Xj,
+                        const double *Mj, double *Lj, const double *Xperiodic, int mutual);
+/* kernel::L2L(Ci, C0), kernels/LaplaceSphericalCPU.cxx:193-224: L += the
+ * parent's local expansion Lp (centre Xp) shifted to X; in MASS mode L is
+ * first divided by the cell's own M0 (M, read only then). */
+int exafmm_hip_cell_l2l(exafmm_hip_handle *h, const double *Xp, const double *Lp, const double *X, const double *M,
+                        double *L);
+/* kernel::L2P(Ci), kernels/LaplaceSphericalCPU.cxx:226-256: for each of n
+ * bodies, trg = trg / q + (potential, gradient) of L about X. */
+int exafmm_hip_cell_l2p(exafmm_hip_handle *h, int n, const double *xyzq, const double *X, const double *L,
+                        double *trg);
 
 #ifdef __cplusplus
 }

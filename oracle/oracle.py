@@ -225,3 +225,31 @@ def read_ref_dump(path):
     out["sample_ibody"] = samp["IBODY"].copy()
     out["sample_direct"] = samp["TRG"].copy()
     return out
+
+
+def read_kernel_chain_dump(path):
+    """Read a dump of oracle/kernel_chain.cxx (the per-cell kernel API chain):
+    M of the 18 cells, L of the two parents after the mutual M2L and of the 8
+    target children after L2L, and the bodies' TRG after L2P, after the mutual
+    P2P (targets, then sources), after the self P2P, and the normalised P2P."""
+    with open(path, "rb") as f:
+        b = f.read()
+    nt, nb = (int(v) for v in np.frombuffer(b, np.int32, 2, 0))
+    off = 8
+    out = {}
+
+    def take(shape):
+        nonlocal off
+        k = int(np.prod(shape))
+        v = np.frombuffer(b, np.float64, k, off).reshape(shape).copy()
+        off += 8 * k
+        return v
+
+    out["M"] = take((18, nt, 2))
+    out["L_parents"] = take((2, nt, 2))
+    out["L_children"] = take((8, nt, 2))
+    n = 8 * nb
+    for k in ("trg_l2p", "trg_p2p_targets", "trg_p2p_sources", "trg_p2p_self", "trg_p2p_normalised"):
+        out[k] = take((n, 4))
+    assert off == len(b), "unexpected dump length"
+    return out

@@ -6,7 +6,7 @@ against the reference sources) and stores their inputs and outputs as small
 .npz files.  The fixtures are data -- inputs and expected outputs -- and are
 what the CPU tests pin the oracle against.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [fixture names]
 """
 import os
 import subprocess
@@ -57,6 +57,19 @@ CASES = [
      "tree"),
     ("cube4k_u_p6_cart_mut", "harness_u_p6_cart", ["gen", "c", "4096"], [0.4, 64, 1000, 1, 0, 1, 0], "full"),
     ("cube2k_u_p6_cart_img3", "harness_u_p6_cart", ["gen", "c", "2048"], [0.4, 64, 1000, 1, 0, 0, 3], "tree"),
+    # The reference Plummer set (dataset.h:108-135, MASS charges 1/N, useRopt 1)
+    # at the expansion orders of the mixed-precision sweep.
+    ("mass_plummer16k_a_p4", "harness_a_p4_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 0, 0], "tree"),
+    ("mass_plummer16k_a_p8", "harness_a_p8_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 0, 0], "tree"),
+    ("mass_plummer16k_a_p12", "harness_a_p12_mass", ["gen", "p", "16384"], [0.4, 64, 1000, 1, 1, 0, 0], "tree"),
+]
+
+# The per-cell kernel API chain (oracle/kernel_chain.cxx) on the reference's
+# kernels: name, binary.
+KERNEL_CHAIN_CASES = [
+    ("kernel_chain_p10", "kernel_chain_p10"),
+    ("kernel_chain_p10_mass", "kernel_chain_p10_mass"),
+    ("kernel_chain_p6_cart", "kernel_chain_p6_cart"),
 ]
 
 
@@ -117,9 +130,21 @@ def unit_cube(n, seed):
     return np.concatenate([x, q[:, None]], axis=1)
 
 
-def main():
+def main(only=None):
+    def wanted(name):
+        return only is None or name in only
+
     with tempfile.TemporaryDirectory() as td:
+        for name, binary in KERNEL_CHAIN_CASES:
+            if not wanted(name):
+                continue
+            out = os.path.join(td, name + ".bin")
+            print(binary, flush=True)
+            subprocess.run([os.path.join(REF, binary), out], check=True)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **O.read_kernel_chain_dump(out))
         for name, binary, mode, opts, keep in CASES:
+            if not wanted(name):
+                continue
             out = os.path.join(td, name + ".bin")
             args = [os.path.join(REF, binary)]
             if mode[0] == "file":
@@ -163,6 +188,8 @@ def main():
                 d["L"] = c["L"]
             np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
         for name, binary, (kind, size, seed), opts in EWALD_CASES:
+            if not wanted(name):
+                continue
             x = jittered_lattice(size, seed) if kind == "lattice" else periodic_cube(size, seed)
             inp = os.path.join(td, name + ".in")
             out = os.path.join(td, name + ".bin")
@@ -178,6 +205,8 @@ def main():
                                 nspawn=opts[2], images=opts[3], ksize=opts[4], alpha=opts[5], sigma=opts[6],
                                 cutoff=opts[7], cycle=opts[8], P=int(binary.split("_p")[1].split("_")[0]),
                                 basis=1 if binary.endswith("_cart") else 0, **r)
+    if not wanted("generators"):
+        return
     # Generator fixtures: drand48 and the first bodies of every distribution.
     g = dict(drand48_seed0=O.drand48_seq(0, 16), drand48_seed42=O.drand48_seq(42, 16))
     for dist in "cspol":
@@ -195,4 +224,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # Optional: the fixture names to (re)write, e.g. kernel_chain_p10; all if none.
+    main(set(sys.argv[1:]) or None)

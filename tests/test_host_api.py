@@ -19,7 +19,8 @@ HEADERS = ["args.h", "bound_box.h", "build_tree_omp.h", "build_tree.h", "dataset
            "thread.h", "traversal.h", "types.h", "up_down_pass.h", "verify.h"]
 
 
-def test_host_headers_compile_and_link(tmp_path):
+@pytest.mark.parametrize("basis", [["-DSpherical", "-DEXPANSION=10"], ["-DCartesian", "-DEXPANSION=6"]])
+def test_host_headers_compile_and_link(tmp_path, basis):
     src = tmp_path / "all.cxx"
     src.write_text("".join(f'#include "{h}"\n' for h in HEADERS) +
                    "int main() { Args a; Dataset d; Bodies b = d.initBodies(100, \"cube\"); Verify v;\n"
@@ -29,7 +30,7 @@ def test_host_headers_compile_and_link(tmp_path):
                    "  up.dipoleCorrection(b, dp, (int)b.size(), 6.28);\n"
                    "  return b.size() == 100 ? 0 : 1; }\n")
     exe = tmp_path / "all"
-    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-DSpherical", "-DFP64", "-DEXPANSION=10", f"-I{HOST}",
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall"] + basis + ["-DFP64", f"-I{HOST}",
                         str(src), f"-L{os.path.join(ROOT, 'exafmm-dev_amd')}", "-lexafmm_hip", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -51,6 +52,29 @@ def test_reference_driver_runs_on_gpu(args):
     if not (os.path.exists(hip_exe) and os.path.exists(cpu_exe)):
         pytest.skip("reference driver binaries are built only where the reference sources exist")
     print("[gpu] serial.cxx on the host API", " ".join(args), flush=True)
+    g = subprocess.run([hip_exe, "-v", "1", "-T", "1"] + args, capture_output=True, text=True, timeout=300)
+    assert g.returncode == 0, g.stderr
+    c = subprocess.run([cpu_exe, "-v", "1", "-T", "1"] + args, capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert c.returncode == 0, c.stderr
+    gp, ga = _errors(g.stdout)
+    cp, ca = _errors(c.stdout)
+    assert abs(gp - cp) <= 1e-6 * cp + 1e-13, (gp, cp)
+    assert abs(ga - ca) <= 1e-6 * ca + 1e-13, (ga, ca)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [["-n", "16384", "-c", "64", "-o", "0", "-m", "0", "-d", "c"],
+                                  ["-n", "8192", "-c", "32", "-o", "0", "-m", "1", "-d", "s"]])
+def test_reference_driver_runs_on_gpu_cartesian(args):
+    """examples/serial.cxx with -DCartesian -DEXPANSION=6 on the host API
+    (the Taylor basis of kernels/LaplaceCartesianCPU.cxx) reports the same
+    FMM-vs-direct errors as the reference's CPU build of the same driver."""
+    hip_exe = os.path.join(ROOT, "oracle", "_ref", "serial_hip_p6_cart")
+    cpu_exe = os.path.join(ROOT, "oracle", "_ref", "serial_p6_cart")
+    if not (os.path.exists(hip_exe) and os.path.exists(cpu_exe)):
+        pytest.skip("reference driver binaries are built only where the reference sources exist")
+    print("[gpu] serial.cxx (Cartesian P=6) on the host API", " ".join(args), flush=True)
     g = subprocess.run([hip_exe, "-v", "1", "-T", "1"] + args, capture_output=True, text=True, timeout=300)
     assert g.returncode == 0, g.stderr
     c = subprocess.run([cpu_exe, "-v", "1", "-T", "1"] + args, capture_output=True, text=True, timeout=300,

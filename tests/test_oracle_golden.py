@@ -16,7 +16,8 @@ CASES = ["cube4k_u_p10", "cube16k_u_p10", "cube16k_a_p10", "plummer16k_a_p10", "
          "octant8k_a_p6", "cube2k_u_p6_periodic", "mass_cube2k_u_p8_img3", "cube4k_a_p10_img3", "unit16k_u_p10", "small100_u_p10", "tiny40_u_p10",
          "cube4k_u_p10_mut", "cube16k_u_p10_mut", "plummer16k_u_p10_mut", "cube16k_a_p10_mut",
          "mass_plummer16k_a_p10_mut", "cube16k_u_p6_cart", "cube16k_a_p6_cart", "cube4k_u_p4_cart",
-         "sphere8k_a_p8_cart", "mass_plummer16k_a_p6_cart", "cube4k_u_p6_cart_mut", "cube2k_u_p6_cart_img3"]
+         "sphere8k_a_p8_cart", "mass_plummer16k_a_p6_cart", "cube4k_u_p6_cart_mut", "cube2k_u_p6_cart_img3",
+         "mass_plummer16k_a_p4", "mass_plummer16k_a_p8", "mass_plummer16k_a_p12"]
 
 
 def test_drand48_and_generators(golden_dir):
