@@ -24,7 +24,7 @@ EXPORTS = (
     "exafmm_hip_get_bodies", "exafmm_hip_get_results", "exafmm_hip_results_device", "exafmm_hip_get_list_counts",
     "exafmm_hip_get_lists", "exafmm_hip_direct", "exafmm_hip_get_stats", "exafmm_hip_synchronize",
     "exafmm_hip_get_dipole", "exafmm_hip_dipole_correction", "exafmm_hip_ewald", "exafmm_hip_count_p2p_bodies",
-    "exafmm_hip_evaluate_batch",
+    "exafmm_hip_evaluate_batch", "exafmm_hip_direct_arrays_static",
     "exafmm_hip_cell_p2p", "exafmm_hip_cell_p2p_self", "exafmm_hip_cell_p2m", "exafmm_hip_cell_m2m",
     "exafmm_hip_cell_m2l", "exafmm_hip_cell_l2l", "exafmm_hip_cell_l2p",
 )
@@ -97,6 +97,7 @@ def load():
     L.exafmm_hip_get_lists.argtypes = [vp, vp, vp, vp, vp]
     L.exafmm_hip_direct.argtypes = [vp, ip, vp, vp]
     L.exafmm_hip_direct_arrays.argtypes = [vp, ip, vp, ip, vp, vp]
+    L.exafmm_hip_direct_arrays_static.argtypes = [vp, ip, vp, ip, vp, vp]
     L.exafmm_hip_cell_p2p.argtypes = [vp, ip, vp, vp, ip, vp, vp, dp, vp, ip]
     L.exafmm_hip_cell_p2p_self.argtypes = [vp, ip, vp, vp, dp]
     L.exafmm_hip_cell_p2m.argtypes = [vp, ip, vp, vp, vp]
@@ -264,6 +265,16 @@ class FMM:
         out = np.zeros((t.shape[0], 4))
         self._check(self.lib.exafmm_hip_direct_arrays(self.h, t.shape[0], _ptr(t), s.shape[0], _ptr(s), _ptr(out)),
                     "direct_arrays")
+        return out
+
+    def direct_arrays_static(self, targets, sources):
+        """direct_arrays through the allocation-free entry point (staging in
+        the library's static device array; 2 nt + ns bodies at most)."""
+        t = np.ascontiguousarray(targets, np.float64)
+        s = np.ascontiguousarray(sources, np.float64)
+        out = np.zeros((t.shape[0], 4))
+        self._check(self.lib.exafmm_hip_direct_arrays_static(self.h, t.shape[0], _ptr(t), s.shape[0], _ptr(s),
+                                                             _ptr(out)), "direct_arrays_static")
         return out
 
     def dipole(self, X0=(0.0, 0.0, 0.0)):

@@ -146,6 +146,19 @@ int exafmm_hip_direct(exafmm_hip_handle *h, int nt, const int *target_ibody, dou
  * by the target charge. */
 int exafmm_hip_direct_arrays(exafmm_hip_handle *h, int nt, const double *t_xyzq, int ns, const double *s_xyzq,
                              double *trg);
+/* exafmm_hip_direct_arrays without any device allocation (no counterpart in
+ * the reference): for callers that cannot allocate device memory per call --
+ * tight serving or verification loops, harnesses that forbid hipMalloc.
+ * The device staging of targets, sources and results lives in one static
+ * device array per GPU of EXAFMM_HIP_STATIC_SCRATCH_BODIES (x, y, z, q)
+ * entries (16 MiB, held until the process ends); a call needs
+ * 2 * nt + ns <= that and returns EXAFMM_HIP_EINVAL otherwise.  Results are
+ * identical to exafmm_hip_direct_arrays.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (from several handles or
+ * threads) race and must be serialised by the caller. */
+#define EXAFMM_HIP_STATIC_SCRATCH_BODIES (1 << 19)
+int exafmm_hip_direct_arrays_static(exafmm_hip_handle *h, int nt, const double *t_xyzq, int ns,
+                                    const double *s_xyzq, double *trg);
 
 /* UpDownPass::getDipole (include/up_down_pass.h:151-157): dipole of the
  * bodies about X0 (NULL = origin), sum of (X - X0) q.  Needs the tree. */
