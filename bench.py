@@ -44,6 +44,26 @@ def unit_cube(n, seed):
     return np.concatenate([x, q[:, None]], axis=1)
 
 
+# How oracle/Makefile builds the two CPU baselines.
+REF_FLAGS = "g++ -O3 -fopenmp -DOPENMP -DSpherical -DFP64 -DEXPANSION=10 (examples/serial.cxx + kernels/*CPU.cxx)"
+ORACLE_FLAGS = "g++ -O2 -std=c++17 -fopenmp -ffp-contract=off (oracle/oracle.cpp)"
+
+
+def host_info():
+    """CPU model and hardware threads of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "threads_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def cpu_baseline(n, P):
     """Time the reference's own driver (examples/serial.cxx built from the
     reference sources into oracle/_ref/) on this host, one thread."""
@@ -68,7 +88,33 @@ def cpu_baseline(n, P):
         "sample": (f"reference examples/serial.cxx (OpenMP build, uniform tree, mutual traversal), "
                    f"N={n} cube, P={P}, theta=0.4, ncrit=64, useRopt=0, 1 thread; Total FMM {t:.2f} s "
                    f"(wall {wall:.1f} s), its rel-L2 acc error {float(acc.group(1)) if acc else float('nan'):.2e}"),
+        "compiler_flags": REF_FLAGS,
+        "host": host_info(),
     }
+
+
+def cpu_baseline_all_cores(x, P):
+    """The same evaluation by the CPU restatement of the reference path
+    (oracle/), its kernel calls spread over every host thread (target cells
+    in parallel, each in the reference's order; passes level by level): the
+    all-cores CPU figure beside the reference's single-core one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import oracle
+    except Exception:
+        return None
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    o = oracle.Oracle(P=P, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, threads=threads)
+    o.set_bodies(x)
+    t0 = time.perf_counter()
+    o.run()
+    t = time.perf_counter() - t0
+    n = x.shape[0]
+    return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": (f"CPU restatement of the reference path (oracle/oracle.cpp, bit-identical trees and lists, "
+                       f"results to 1e-12), N={n} cube, P={P}, theta=0.4, ncrit=64, non-mutual, {threads} threads; "
+                       f"bounds to downward pass {t:.2f} s"),
+            "compiler_flags": ORACLE_FLAGS, "host": host_info()}
 
 
 def measured_traffic(kernel):
@@ -109,9 +155,12 @@ def rel_err(res, ref):
 def bench_batched(args):
     """256 independent systems, N log-uniform in [4096, 262144], half uniform
     cube, half uniform on the unit sphere surface, zero-mean charges; P = 8,
-    theta = 0.5; ncrit autotuned over {32, 64, 128}.  value = total bodies /
-    device time of one batched evaluation (HIP events, inputs resident);
-    compared with one system of the same total N (uniform cube)."""
+    theta = 0.5; ncrit, the P2P grid and targets per lane and the contraction
+    grid chosen by the library's autotuner (exafmm_hip_set_autotune).  value =
+    total bodies / device time of one batched evaluation (HIP events); the
+    wall time per call including the host copies of inputs and results is
+    reported beside it; compared with one system of the same total N (uniform
+    cube, tuned the same way)."""
     rng = np.random.default_rng(2024)
     sizes = np.floor(np.exp(rng.uniform(np.log(4096), np.log(262144), 256))).astype(np.int64)
     systems = []
@@ -126,24 +175,23 @@ def bench_batched(args):
         systems.append(np.concatenate([x, q[:, None]], axis=1))
     total = int(sizes.sum())
     steps = max(1, args.steps)
-    tune = {}
-    best = None
-    for ncrit in (32, 64, 128):
-        f = exafmm_hip.FMM(P=8, theta=0.5, ncrit=ncrit)
-        out = f.evaluate_batch(systems)  # warm-up
-        dev, wall = [], []
-        for _ in range(steps):
-            t0 = time.perf_counter()
-            out = f.evaluate_batch(systems)
-            wall.append((time.perf_counter() - t0) * 1e3)
-            dev.append(f.stats()["t_total"])
-        ms = float(np.median(dev))
-        tune[ncrit] = {"ms_device": round(ms, 3), "ms_wall_incl_host_copies": round(float(np.median(wall)), 3)}
-        if best is None or ms < best[1]:
-            best = (ncrit, ms, out, f)
-        else:
-            f.close()
-    ncrit, ms, out, f = best
+    # The library's autotuner picks ncrit, the P2P grid and targets per lane
+    # and the contraction grid on the first call of this input signature.
+    f = exafmm_hip.FMM(P=8, theta=0.5, ncrit=64)
+    f.set_autotune(True)
+    t0 = time.perf_counter()
+    out = f.evaluate_batch(systems)
+    tune_s = time.perf_counter() - t0
+    tuned = f.tuning()
+    ncrit = tuned["ncrit"]
+    dev, wall = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        out = f.evaluate_batch(systems)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(f.stats()["t_total"])
+    ms = float(np.median(dev))
+    ms_wall = float(np.median(wall))
     # Per-system error against FP64 direct summation on 64 sampled targets.
     errs = []
     for x, r in zip(systems, out):
@@ -154,6 +202,7 @@ def bench_batched(args):
     f.close()
     # One system of the same total N.
     g = exafmm_hip.FMM(P=8, theta=0.5, ncrit=ncrit)
+    g.set_autotune(True)  # the single system gets its own tuning
     g.set_bodies(unit_cube(total, 99))
     g.evaluate()
     sdev = []
@@ -170,7 +219,8 @@ def bench_batched(args):
         "config": {"workload": "batched evaluation, one forest of per-system uniform trees", "systems": len(sizes),
                    "total_N": total, "N_min": int(sizes.min()), "N_max": int(sizes.max()), "P": 8, "theta": 0.5,
                    "ncrit": ncrit},
-        "autotune_ncrit": tune,
+        "autotune": {"chosen": tuned, "first_call_s_incl_tuning": round(tune_s, 3)},
+        "ms_wall_incl_host_copies": round(ms_wall, 3),
         "single_system_same_N": {"value": total / (sms * 1e-3), "ms": sms, "workload": "uniform cube, same N, ncrit"},
         "ratio_vs_single_system": (total / (ms * 1e-3)) / (total / (sms * 1e-3)),
         "rel_l2_err_per_system": {"pot_max": float(errs[:, 0].max()), "acc_max": float(errs[:, 1].max()),
@@ -389,6 +439,7 @@ def main():
                                            "frac": round(a / kern[k][3], 5)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, P)
+    cpu_all = None if args.no_cpu_baseline else cpu_baseline_all_cores(x, P)
     line = {
         "metric": METRIC,
         "value": value,
@@ -415,6 +466,7 @@ def main():
         "roofline": roofline,
         "mixed": mixed,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(line), flush=True)
 

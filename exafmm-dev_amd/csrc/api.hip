@@ -150,6 +150,8 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_wpc = std::atoi(m);  // per CU
+  if (const char *m = std::getenv("EXAFMM_HIP_P2P_KERNEL")) H->h.p2p_kernel = std::atoi(m);
+  if (const char *m = std::getenv("EXAFMM_HIP_P2P_KMAX")) H->h.p2p_kmax = std::atoi(m);
   if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
   // Rows of the lattice T(d) table kept across evaluations (16384 rows are
   // 0.5 GB in FP64 at P = 10).
@@ -315,8 +317,94 @@ static void evaluate_all(Handle &h) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Autotuning.  The signature of an input: the settings that change the work
+// (P, theta, precision, builder, basis, images, mass, useRopt), the system
+// count, a histogram of the systems' sizes by powers of two and the total
+// size to ~6%; inputs with one signature share their tuned parameters.
+static std::vector<int64_t> tune_signature(const Handle &h) {
+  std::vector<int64_t> k = {h.cfg.P, (int64_t)std::llround(h.cfg.theta * 1e6), h.cfg.precision, h.cfg.builder,
+                            h.cfg.basis, h.cfg.images, h.cfg.mass, h.cfg.useRopt, h.nsys};
+  std::vector<int64_t> hist(32, 0);
+  for (int s = 0; s < h.nsys; s++) {
+    const int n = h.nsys > 1 ? h.sys_off[s + 1] - h.sys_off[s] : h.n;
+    int b = 0;
+    while ((1 << (b + 1)) <= n && b < 30) b++;
+    hist[b]++;
+  }
+  k.insert(k.end(), hist.begin(), hist.end());
+  int64_t q = std::max<int64_t>(1, h.n);
+  int e = 0;
+  while (q >= 32) {  // the total to 5 significant bits
+    q >>= 1;
+    e++;
+  }
+  k.push_back(q);
+  k.push_back(e);
+  return k;
+}
+
+static void apply_tuned(Handle &h, const Handle::Tuned &t) {
+  if (t.ncrit != h.cfg.ncrit) h.have_tree = h.have_up = h.have_trav = false;
+  h.cfg.ncrit = t.ncrit;
+  h.p2p_wpc = t.p2p_wpc;
+  h.p2p_kmax = t.p2p_kmax;
+  h.lat_wg_cu = t.lat_wg_cu;
+}
+
+// One evaluation with the tuned parameters of the input's signature; an
+// unseen signature is tuned first by timing candidate evaluations (device
+// time, best of two each): ncrit in {32, 64, 128}, then the P2P grid (8, 12,
+// 16 waves per CU), the P2P targets per lane (3, 4, 5) and the contraction
+// grid (one workgroup per item, or 2 per CU looping over the items), each
+// step keeping the best so far.
+static void evaluate_tuned(Handle &h) {
+  if (!h.autotune) return evaluate_all(h);
+  const std::vector<int64_t> key = tune_signature(h);
+  auto it = h.tune_cache.find(key);
+  if (it == h.tune_cache.end()) {
+    Handle::Tuned best{h.cfg.ncrit, h.p2p_wpc, h.p2p_kmax, h.lat_wg_cu, 1e300};
+    auto trial = [&](Handle::Tuned t) {
+      apply_tuned(h, t);
+      double ms = 1e300;
+      for (int r = 0; r < 2; r++) {
+        evaluate_all(h);
+        ms = std::min(ms, h.stats.t_total);
+      }
+      if (ms < best.ms) {
+        best = t;
+        best.ms = ms;
+      }
+    };
+    for (int nc : {32, 64, 128}) trial({nc, best.p2p_wpc, best.p2p_kmax, best.lat_wg_cu, 0});
+    const Handle::Tuned base = best;
+    for (int w : {8, 16}) trial({base.ncrit, w, base.p2p_kmax, base.lat_wg_cu, 0});
+    const Handle::Tuned b2 = best;
+    for (int k : {3, 5}) trial({b2.ncrit, b2.p2p_wpc, k, b2.lat_wg_cu, 0});
+    const Handle::Tuned b3 = best;
+    trial({b3.ncrit, b3.p2p_wpc, b3.p2p_kmax, b3.lat_wg_cu == 0 ? 2 : 0, 0});
+    it = h.tune_cache.emplace(key, best).first;
+  }
+  apply_tuned(h, it->second);
+  evaluate_all(h);
+}
+
 int exafmm_hip_evaluate(exafmm_hip_handle *H) {
-  return guarded(H, [&](Handle &h) { evaluate_all(h); });
+  return guarded(H, [&](Handle &h) { evaluate_tuned(h); });
+}
+
+int exafmm_hip_set_autotune(exafmm_hip_handle *H, int enable) {
+  return guarded(H, [&](Handle &h) { h.autotune = enable != 0; });
+}
+
+int exafmm_hip_get_tuning(exafmm_hip_handle *H, int *ncrit, int *p2p_waves_per_cu, int *p2p_targets_per_lane,
+                          int *m2l_groups_per_cu) {
+  return guarded(H, [&](Handle &h) {
+    if (ncrit) *ncrit = h.cfg.ncrit;
+    if (p2p_waves_per_cu) *p2p_waves_per_cu = h.p2p_wpc;
+    if (p2p_targets_per_lane) *p2p_targets_per_lane = h.p2p_kmax;
+    if (m2l_groups_per_cu) *m2l_groups_per_cu = h.lat_wg_cu;
+  });
 }
 
 int exafmm_hip_evaluate_batch(exafmm_hip_handle *H, int nsys, const int *n, const double *xyzq, double *trg) {
@@ -337,13 +425,12 @@ int exafmm_hip_evaluate_batch(exafmm_hip_handle *H, int nsys, const int *n, cons
     h.nsys = nsys;
     h.sys_off = off;
     h.have_bounds = h.have_tree = h.have_up = h.have_trav = false;
-    evaluate_all(h);
-    DBuf<double4> o;
-    o.resize(total);
+    evaluate_tuned(h);
+    h.out4.resize(total);
     hipLaunchKernelGGL(k_scatter_results, dim3(div_up(total, 256)), dim3(256), 0, h.stream, h.trg4.p, h.bidx.p, total,
-                       o.p);
+                       h.out4.p);
     EXA_HIP(hipGetLastError());
-    EXA_HIP(hipMemcpyAsync(trg, o.p, (size_t)total * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipMemcpyAsync(trg, h.out4.p, (size_t)total * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
   });
 }
@@ -396,13 +483,12 @@ int exafmm_hip_get_bodies(exafmm_hip_handle *H, double *xyzq, int *ibody, double
 int exafmm_hip_get_results(exafmm_hip_handle *H, double *trg) {
   return guarded(H, [&](Handle &h) {
     ensure(h.have_trav && trg, "get_results: nothing evaluated");
-    DBuf<double4> o;
-    o.resize(h.n);
-    hipLaunchKernelGGL(k_scatter_results, dim3(div_up(h.n, 256)), dim3(256), 0, h.stream, h.trg4.p, h.bidx.p, h.n, o.p);
+    h.out4.resize(h.n);
+    hipLaunchKernelGGL(k_scatter_results, dim3(div_up(h.n, 256)), dim3(256), 0, h.stream, h.trg4.p, h.bidx.p, h.n,
+                       h.out4.p);
     EXA_HIP(hipGetLastError());
-    EXA_HIP(hipMemcpyAsync(trg, o.p, h.n * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipMemcpyAsync(trg, h.out4.p, h.n * sizeof(double4), hipMemcpyDeviceToHost, h.stream));
     EXA_HIP(hipStreamSynchronize(h.stream));
-    o.release();
   });
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -146,6 +147,8 @@ struct Handle {
   // workgroups per CU looping over (tile, segment) items (0: one workgroup
   // per item).  ncu = compute units.
   int p2p_wpc = 12, lat_wg_cu = 0, ncu = 0;
+  int p2p_kernel = 1;  // P2P: 1 one target per lane (k_p2p_f64 / k_p2p_mixed), 0 several (the _rb kernels)
+  int p2p_kmax = 4;    // k_p2p_f64_rb: the most targets per lane (3, 4 or 5)
   DBuf<double> ud_ops;
   DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
   DBuf<double> lat_part;
@@ -201,6 +204,19 @@ struct Handle {
   DBuf<unsigned char> co_img;
 
   exafmm_hip_stats stats = {};
+
+  // Caller-order results staged on the device (get_results, evaluate_batch).
+  DBuf<double4> out4;
+
+  // Autotuning (exafmm_hip_set_autotune): for each input signature, the
+  // fastest (ncrit, P2P waves per CU, P2P targets per lane, contraction
+  // workgroups per CU) among timed candidate evaluations.
+  struct Tuned {
+    int ncrit, p2p_wpc, p2p_kmax, lat_wg_cu;
+    double ms;
+  };
+  bool autotune = false;
+  std::map<std::vector<int64_t>, Tuned> tune_cache;
 
   ~Handle();
 };

@@ -69,10 +69,11 @@ struct SrcDesc {
   unsigned char img[64];
 };
 
+// skip: a cell whose unshifted entry (image 13) is left out (-1: none).
 template <int STAGE, typename Put>
 __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__restrict__ src,
                                  const unsigned char *__restrict__ img, const int *__restrict__ cBody,
-                                 const int *__restrict__ cNBody, SrcDesc &sd, Put put) {
+                                 const int *__restrict__ cNBody, SrcDesc &sd, Put put, int skip = -1) {
   __syncthreads();  // the previous stage (and descriptor batch) are consumed
   if (w.e >= w.dbase + w.dcount) {
     w.dbase = w.e;
@@ -80,9 +81,10 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
     int n = 0;
     if (lane < w.dcount) {
       const int s = src[w.e + lane];
+      const unsigned char im = img[w.e + lane];
       sd.b0[lane] = cBody[s];
-      n = cNBody[s];
-      sd.img[lane] = img[w.e + lane];
+      n = (s == skip && im == 13) ? 0 : cNBody[s];
+      sd.img[lane] = im;
     }
     int incl = n;
     for (int d = 1; d < 64; d <<= 1) {
@@ -292,6 +294,174 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 P2P, register-blocked: every lane holds K targets, so each source
+// body read from LDS serves K pairs (the one-target layout above reads a
+// source per pair, which kept the LDS return path close to saturation), and
+// the next source is read into registers while the current one is evaluated
+// (the loop never waits on a read it has just issued).
+//
+// Lane layout: lane = 8 g + i (g = 0..7 groups, i = 0..7) holds targets
+// i, i + 8, ..., i + 8 (K - 1) of a block of n <= 8 KMAX targets,
+// K = ceil(n / 8) (a compile-time constant per block), and takes the staged
+// sources g, g + 8, g + 16, ...; the 8 group partials of a target are summed
+// in group order at the end of the block (deterministic).  With ~32 bodies
+// per leaf this wastes ~10% of the pair slots, as the two-pass layout does.
+//
+// The target leaf's own bodies form a masked stage sequence (R2 == 0 is the
+// self pair, skipped by a select); the list walk leaves the own entry out, so
+// every other stage runs without the select (distinct leaves never hold two
+// bodies at one position: equal positions share a Morton key and a leaf),
+// except stages with periodic images, which keep it.
+constexpr int kRbStage = 256;  // sources staged per fill
+constexpr int kRbG = 8;        // source groups (lanes per group: 8)
+
+template <bool MASK>
+__device__ inline void rb_pair(double xi, double yi, double zi, const double4 &xj, double4 &acc) {
+  const double dx = xi - xj.x, dy = yi - xj.y, dz = zi - xj.z;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double rs = rsqrt_pos(r2);
+  const double invr = MASK ? (r2 > 0 ? rs : 0.0) : rs;
+  const double qinvr = xj.w * invr;
+  const double qinvr3 = qinvr * invr * invr;
+  acc.x += qinvr;
+  acc.y += dx * qinvr3;
+  acc.z += dy * qinvr3;
+  acc.w += dz * qinvr3;
+}
+
+// One tile: sources sg[0], sg[8], ..., sg[ncp - 8] (ncp a multiple of 8);
+// the read-ahead touches sg[ncp], which the caller pads.
+template <bool MASK, int K>
+__device__ inline void rb_tile(const double4 *__restrict__ sg, int ncp, const double (&x)[K], const double (&y)[K],
+                               const double (&z)[K], double4 (&acc)[K]) {
+  double4 cur = sg[0];
+  for (int j = 0; j < ncp; j += kRbG) {
+    const double4 nxt = sg[j + kRbG];
+#pragma unroll
+    for (int k = 0; k < K; k++) rb_pair<MASK>(x[k], y[k], z[k], cur, acc[k]);
+    cur = nxt;
+  }
+}
+
+template <int K>
+__device__ void rb_block(int lane, int c, int tb0, int tbase, int nt, int e0, int e1, const int *__restrict__ src,
+                         const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                         const int *__restrict__ cNBody, const double4 *__restrict__ body, double4 *__restrict__ trg,
+                         double cycle, int tn, double4 *sb, SrcDesc &sd) {
+  const int g = lane / kRbG, i = lane % kRbG;
+  double x[K], y[K], z[K];
+  double4 acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int t = i + kRbG * k;
+    // Absent targets sit far from everything (their results are dropped).
+    const double4 v = t < nt ? body[tb0 + tbase + t] : make_double4(-kFar, -kFar, -kFar, 0);
+    x[k] = v.x;
+    y[k] = v.y;
+    z[k] = v.z;
+    acc[k] = make_double4(0, 0, 0, 0);
+  }
+  // Pad the tile to a multiple of 8, plus 8 for the read-ahead, with zero
+  // charges far away (they add exactly zero), then run it.
+  auto run = [&](int cnt, bool masked) __attribute__((always_inline)) {
+    const int cntp = (cnt + kRbG - 1) / kRbG * kRbG;
+    for (int k = cnt + lane; k < cntp + kRbG; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
+    __syncthreads();
+    const int ncp = __builtin_amdgcn_readfirstlane(cntp);
+    if (masked)
+      rb_tile<true, K>(sb + g, ncp, x, y, z, acc);
+    else
+      rb_tile<false, K>(sb + g, ncp, x, y, z, acc);
+  };
+  // The own leaf's bodies, masked.
+  for (int s0 = 0; s0 < tn; s0 += kRbStage) {
+    const int cnt = min(kRbStage, tn - s0);
+    __syncthreads();
+    for (int k = lane; k < cnt; k += 64) sb[k] = body[tb0 + s0 + k];
+    run(cnt, true);
+  }
+  // The rest of the list, the own entry left out.
+  SrcWalker w{e0, 0, e0, 0};
+  while (w.e < e1) {
+    bool imaged = false;
+    const int cnt = fill_stage<kRbStage>(
+        w, e1, lane, src, img, cBody, cNBody, sd,
+        [&](int k, int b, int im) {
+          double px, py, pz;
+          image_shift(im, cycle, px, py, pz);
+          double4 v = body[b];
+          v.x += px;
+          v.y += py;
+          v.z += pz;
+          sb[k] = v;
+          imaged |= im != 13;
+        },
+        c);
+    if (cnt > 0) run(cnt, __any(imaged));
+  }
+  // Group partials of target i + 8 k in slots 8 g + i: summed in group order.
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    __syncthreads();
+    sb[lane] = acc[k];
+    __syncthreads();
+    const int t = lane + kRbG * k;
+    if (lane < kRbG && t < nt) {
+      double4 s = sb[lane];
+      for (int gg = 1; gg < 64 / kRbG; gg++) {
+        const double4 o = sb[gg * kRbG + lane];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tbase + t];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tbase + t] = tv;
+    }
+  }
+}
+
+// KMAX: the most targets per lane; a leaf of more than 8 KMAX bodies runs as
+// several blocks (each walks its list once).
+template <int KMAX>
+__global__ void __launch_bounds__(64, 4)
+    k_p2p_f64_rb(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
+                 int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
+                 const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                 const int *__restrict__ cNBody, const double4 *__restrict__ body, double4 *__restrict__ trg,
+                 double cycle) {
+  __shared__ double4 sb[kRbStage + 2 * kRbG];
+  __shared__ SrcDesc sd;
+  const int lane = threadIdx.x;
+  const int ntl = ntl_dev ? *ntl_dev : ntl_host;
+  for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
+    const int c = tgt_list[bi];
+    const int tb0 = cBody[c], tn = cNBody[c];
+    const int e0 = off[c], e1 = off[c + 1];
+    for (int tbase = 0; tbase < tn; tbase += kRbG * KMAX) {
+      const int nt = min(kRbG * KMAX, tn - tbase);
+      const int K = (nt + kRbG - 1) / kRbG;
+#define EXA_RB_BLOCK(k) rb_block<k>(lane, c, tb0, tbase, nt, e0, e1, src, img, cBody, cNBody, body, trg, cycle, tn, sb, sd)
+      if (K <= 1)
+        EXA_RB_BLOCK(1);
+      else if (K == 2)
+        EXA_RB_BLOCK(2);
+      else if (K == 3 || KMAX == 3)
+        EXA_RB_BLOCK(3 <= KMAX ? 3 : KMAX);
+      else if (K == 4 || KMAX == 4)
+        EXA_RB_BLOCK(4 <= KMAX ? 4 : KMAX);
+      else
+        EXA_RB_BLOCK(KMAX);
+#undef EXA_RB_BLOCK
+    }
+  }
+}
+
 // Mixed precision (cfg.precision = 1): FP32 pair arithmetic on positions
 // taken relative to the target leaf's centre in FP64 (the source's image
 // shift folded in) and split into FP32 hi + lo parts, so the FP32 values are
@@ -489,6 +659,192 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision, register-blocked: the lane layout and stage walk of
+// k_p2p_f64_rb (K targets per lane, own leaf masked, read-ahead) with the
+// arithmetic of k_p2p_mixed: leaf-centred FP32 hi + lo positions, runs of
+// kSeg FP32 terms per target, each run added into a compensated
+// accumulator (kahan.h:35-41), read out as s - c.  Tiles are padded to whole
+// runs of every group (kSeg 8 = 64 sources).
+struct RbTgt32 {
+  float x, y, z, xl, yl, zl;
+};
+
+template <bool MASK>
+__device__ inline void rb_pair32(const RbTgt32 &t, const float4 &s, const float4 &sl, float4 &a) {
+  const float dx = (t.x - s.x) + (t.xl - sl.x);
+  const float dy = (t.y - s.y) + (t.yl - sl.y);
+  const float dz = (t.z - s.z) + (t.zl - sl.z);
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float rs = __builtin_amdgcn_rsqf(r2);
+  const float invr = MASK ? (r2 > 0.0f ? rs : 0.0f) : rs;
+  const float qinvr = s.w * invr;
+  const float qinvr3 = qinvr * invr * invr;
+  a.x += qinvr;
+  a.y += dx * qinvr3;
+  a.z += dy * qinvr3;
+  a.w += dz * qinvr3;
+}
+
+// One tile: sources sg[0], sg[8], ... (ncp a multiple of 8 kSeg); each run
+// of kSeg sources of a lane is summed in FP32 per target, then added into
+// the target's compensated sums.
+template <bool MASK, int K>
+__device__ inline void rb_tile32(const float4 *__restrict__ sg, const float4 *__restrict__ sgl, int ncp,
+                                 const RbTgt32 (&t)[K], Kahan32 (&kp)[K][4]) {
+  float4 cur = sg[0], curl = sgl[0];
+  for (int j = 0; j < ncp; j += kRbG * kSeg) {
+    float4 run[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) run[k] = make_float4(0, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < kSeg; u++) {
+      const int jn = j + (u + 1) * kRbG;
+      const float4 nxt = sg[jn], nxtl = sgl[jn];
+#pragma unroll
+      for (int k = 0; k < K; k++) rb_pair32<MASK>(t[k], cur, curl, run[k]);
+      cur = nxt;
+      curl = nxtl;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      kp[k][0].add(run[k].x);
+      kp[k][1].add(run[k].y);
+      kp[k][2].add(run[k].z);
+      kp[k][3].add(run[k].w);
+    }
+  }
+}
+
+template <int K>
+__device__ void rb_block32(int lane, int c, int tb0, int tbase, int nt, int e0, int e1, const int *__restrict__ src,
+                           const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                           const int *__restrict__ cNBody, const double4 &Xc, const double4 *__restrict__ body,
+                           double4 *__restrict__ trg, double cycle, int tn, float4 *sb, float4 *sbl, SrcDesc &sd) {
+  const int g = lane / kRbG, i = lane % kRbG;
+  RbTgt32 t[K];
+  Kahan32 kp[K][4];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int ti = i + kRbG * k;
+    // Absent targets sit at the leaf centre (finite; their results are dropped).
+    const double4 v = ti < nt ? body[tb0 + tbase + ti] : Xc;
+    const double rx = v.x - Xc.x, ry = v.y - Xc.y, rz = v.z - Xc.z;
+    t[k].x = (float)rx;
+    t[k].y = (float)ry;
+    t[k].z = (float)rz;
+    t[k].xl = (float)(rx - t[k].x);
+    t[k].yl = (float)(ry - t[k].y);
+    t[k].zl = (float)(rz - t[k].z);
+  }
+  // Source body b shifted by image im, relative to the leaf centre, into
+  // slot k as FP32 hi (with the charge) and lo parts.
+  auto put = [&](int k, const double4 &v, double px, double py, double pz) __attribute__((always_inline)) {
+    const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
+    const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+    sb[k] = make_float4(hx, hy, hz, (float)v.w);
+    sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
+  };
+  auto run = [&](int cnt, bool masked) __attribute__((always_inline)) {
+    constexpr int step = kRbG * kSeg;
+    const int cntp = (cnt + step - 1) / step * step;
+    for (int k = cnt + lane; k < cntp + kRbG; k += 64) {
+      sb[k] = make_float4(kFarF, kFarF, kFarF, 0);
+      sbl[k] = make_float4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    const int ncp = __builtin_amdgcn_readfirstlane(cntp);
+    if (masked)
+      rb_tile32<true, K>(sb + g, sbl + g, ncp, t, kp);
+    else
+      rb_tile32<false, K>(sb + g, sbl + g, ncp, t, kp);
+  };
+  for (int s0 = 0; s0 < tn; s0 += kRbStage) {
+    const int cnt = min(kRbStage, tn - s0);
+    __syncthreads();
+    for (int k = lane; k < cnt; k += 64) put(k, body[tb0 + s0 + k], 0, 0, 0);
+    run(cnt, true);
+  }
+  SrcWalker w{e0, 0, e0, 0};
+  while (w.e < e1) {
+    bool imaged = false;
+    const int cnt = fill_stage<kRbStage>(
+        w, e1, lane, src, img, cBody, cNBody, sd,
+        [&](int k, int b, int im) {
+          double px, py, pz;
+          image_shift(im, cycle, px, py, pz);
+          put(k, body[b], px, py, pz);
+          imaged |= im != 13;
+        },
+        c);
+    if (cnt > 0) run(cnt, __any(imaged));
+  }
+  double4 *const sacc = reinterpret_cast<double4 *>(sb);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    __syncthreads();
+    sacc[lane] = make_double4(kp[k][0].value(), kp[k][1].value(), kp[k][2].value(), kp[k][3].value());
+    __syncthreads();
+    const int ti = lane + kRbG * k;
+    if (lane < kRbG && ti < nt) {
+      double4 s = sacc[lane];
+      for (int gg = 1; gg < 64 / kRbG; gg++) {
+        const double4 o = sacc[gg * kRbG + lane];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+        s.w += o.w;
+      }
+      double4 tv = trg[tb0 + tbase + ti];
+      tv.x += s.x;
+      tv.y -= s.y;
+      tv.z -= s.z;
+      tv.w -= s.w;
+      trg[tb0 + tbase + ti] = tv;
+    }
+  }
+}
+
+template <int KMAX>
+__global__ void __launch_bounds__(64, 4)
+    k_p2p_mixed_rb(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
+                   int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
+                   const unsigned char *__restrict__ img, const int *__restrict__ cBody,
+                   const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
+                   double4 *__restrict__ trg, double cycle) {
+  // Stage (hi parts and charge, then lo parts), padded to whole runs plus
+  // the read-ahead; the block's partials (64 double4) reuse the hi array.
+  constexpr int kCap = kRbStage + kRbG * kSeg + kRbG;
+  __shared__ __attribute__((aligned(32))) float4 sb[kCap];
+  __shared__ float4 sbl[kCap];
+  __shared__ SrcDesc sd;
+  const int lane = threadIdx.x;
+  const int ntl = ntl_dev ? *ntl_dev : ntl_host;
+  for (int bi = blockIdx.x; bi < ntl; bi = next_leaf(next, lane)) {
+    const int c = tgt_list[bi];
+    const int tb0 = cBody[c], tn = cNBody[c];
+    const double4 Xc = cX[c];
+    const int e0 = off[c], e1 = off[c + 1];
+    for (int tbase = 0; tbase < tn; tbase += kRbG * KMAX) {
+      const int nt = min(kRbG * KMAX, tn - tbase);
+      const int K = (nt + kRbG - 1) / kRbG;
+#define EXA_RB_BLOCK32(k) \
+  rb_block32<k>(lane, c, tb0, tbase, nt, e0, e1, src, img, cBody, cNBody, Xc, body, trg, cycle, tn, sb, sbl, sd)
+      if (K <= 1)
+        EXA_RB_BLOCK32(1);
+      else if (K == 2)
+        EXA_RB_BLOCK32(2);
+      else if (K == 3 || KMAX == 3)
+        EXA_RB_BLOCK32(3 <= KMAX ? 3 : KMAX);
+      else if (K == 4 || KMAX == 4)
+        EXA_RB_BLOCK32(4 <= KMAX ? 4 : KMAX);
+      else
+        EXA_RB_BLOCK32(KMAX);
+#undef EXA_RB_BLOCK32
+    }
+  }
+}
+
 // Target-source body pairs of the P2P list (for the interactions/s metric).
 __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, const int *__restrict__ off,
                                    const int *__restrict__ src, const int *__restrict__ cNBody,
@@ -522,12 +878,23 @@ void run_p2p(Handle &h, hipStream_t st) {
   }
   h.p2p_next.reserve(1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, h.p2p_next.p);
-  if (h.cfg.precision == 1)
+  if (h.cfg.precision == 1 && h.p2p_kernel == 1)
     hipLaunchKernelGGL(k_p2p_mixed, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
                        h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
-  else
+  else if (h.cfg.precision == 1) {
+    auto *kern = h.p2p_kmax == 3 ? k_p2p_mixed_rb<3> : h.p2p_kmax == 5 ? k_p2p_mixed_rb<5> : k_p2p_mixed_rb<4>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p,
+                       h.cfg.cycle);
+  }
+  else if (h.p2p_kernel == 1)
     hipLaunchKernelGGL(k_p2p_f64, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
                        h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+  else {
+    auto *kern = h.p2p_kmax == 3 ? k_p2p_f64_rb<3> : h.p2p_kmax == 5 ? k_p2p_f64_rb<5> : k_p2p_f64_rb<4>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+  }
   EXA_HIP(hipGetLastError());
 }
 

@@ -27,6 +27,7 @@ EXPORTS = (
     "exafmm_hip_evaluate_batch", "exafmm_hip_direct_arrays_static",
     "exafmm_hip_cell_p2p", "exafmm_hip_cell_p2p_self", "exafmm_hip_cell_p2m", "exafmm_hip_cell_m2m",
     "exafmm_hip_cell_m2l", "exafmm_hip_cell_l2l", "exafmm_hip_cell_l2p",
+    "exafmm_hip_set_autotune", "exafmm_hip_get_tuning",
 )
 
 
@@ -105,6 +106,8 @@ def load():
     L.exafmm_hip_cell_m2l.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ip]
     L.exafmm_hip_cell_l2l.argtypes = [vp, vp, vp, vp, vp, vp]
     L.exafmm_hip_cell_l2p.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.exafmm_hip_set_autotune.argtypes = [vp, ip]
+    L.exafmm_hip_get_tuning.argtypes = [vp, P(ip), P(ip), P(ip), P(ip)]
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]
@@ -314,6 +317,20 @@ class FMM:
         self._check(self.lib.exafmm_hip_evaluate_batch(self.h, len(arrs), n.ctypes.data, x.ctypes.data,
                                                        out.ctypes.data), "evaluate_batch")
         return np.split(out, np.cumsum(n)[:-1])
+
+    def set_autotune(self, enable=True):
+        """Tune ncrit, the P2P grid and targets per lane, and the contraction
+        grid per input signature (the first evaluation of a signature times
+        the candidates; later ones reuse the winner)."""
+        self._check(self.lib.exafmm_hip_set_autotune(self.h, int(bool(enable))), "set_autotune")
+
+    def tuning(self):
+        """The parameters in use: ncrit, P2P waves per CU, P2P targets per
+        lane, contraction workgroups per CU (0: one per work item)."""
+        v = [ctypes.c_int(0) for _ in range(4)]
+        self._check(self.lib.exafmm_hip_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "get_tuning")
+        return dict(zip(("ncrit", "p2p_waves_per_cu", "p2p_targets_per_lane", "m2l_groups_per_cu"),
+                        (x.value for x in v)))
 
     def count_p2p_bodies(self):
         """Target-source body pairs of the current P2P list."""

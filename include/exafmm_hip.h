@@ -188,6 +188,22 @@ int exafmm_hip_evaluate_batch(exafmm_hip_handle *h, int nsys, const int *n, cons
 
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 
+/* Autotuning (no counterpart in the reference, whose parameters are fixed
+ * command-line flags, include/args.h:98-100).  When enabled, evaluate and
+ * evaluate_batch run with the fastest configuration found for the input's
+ * signature (the settings that change the work, the system count, the
+ * systems' sizes by powers of two and the total size to ~6%): the first call
+ * with a new signature times candidate evaluations -- ncrit in {32, 64, 128},
+ * P2P grid of 8, 12 or 16 waves per CU, 3, 4 or 5 P2P targets per lane, and
+ * the contraction grid -- and later calls reuse the winner.  ncrit is part of
+ * the FMM (it changes the tree and so the approximation), so a tuned run is
+ * a valid FMM evaluation at the chosen ncrit, not bit-identical to another.
+ * exafmm_hip_get_tuning reports the parameters in use (any pointer may be
+ * NULL). */
+int exafmm_hip_set_autotune(exafmm_hip_handle *h, int enable);
+int exafmm_hip_get_tuning(exafmm_hip_handle *h, int *ncrit, int *p2p_waves_per_cu, int *p2p_targets_per_lane,
+                          int *m2l_groups_per_cu);
+
 /* Target-source body pairs of the current P2P list (a reduction on the
  * device; also stored into the stats' num_p2p_bodies until the next
  * traversal).  Kept out of get_stats so that reading phase times stays cheap.

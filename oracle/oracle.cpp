@@ -188,6 +188,15 @@ struct orc_ctx {
   std::vector<cd> M, L;
   std::vector<int> p2p_i, p2p_j, p2p_img, m2l_i, m2l_j, m2l_img;
   int level = 0;
+  // Deferred evaluation (p.threads != 1, non-mutual, no images): the
+  // traversal only records its kernel calls; afterwards every target cell's
+  // calls run in emission order, target cells in parallel.  A non-mutual call
+  // writes only its target cell (L or the bodies' TRG), so each result is
+  // summed in exactly the serial order and is bit-identical to it.
+  bool deferring = false;
+  std::vector<int> def_ci, def_cj;
+  std::vector<unsigned char> def_kind;  // 0: M2L, 1: P2P, 2: self P2P
+  bool parallel() const { return p.threads != 1; }
 
   CartTables ct;  // Cartesian basis tables (p.basis == 1)
   cd *Mc(int c) { return &M[(size_t)c * nterm]; }
@@ -919,13 +928,30 @@ struct orc_ctx {
     L.assign((size_t)nc * nterm, cd(0));
     if (nc == 0) return;
     // Children always carry larger indices than their parent in both
-    // layouts, so a reverse sweep is a valid post-order.
+    // layouts, so a reverse sweep is a valid post-order.  In parallel mode
+    // the multipoles are formed first, depth by depth (each cell writes only
+    // its own M from its bodies or its children's M: the same sums as the
+    // sweep), then the sweep does the radii.
+    if (parallel()) {
+      std::vector<std::vector<int>> lv = cells_by_depth();
+#pragma omp parallel for schedule(dynamic, 16)
+      for (int c = 0; c < nc; c++)
+        if (cells[c].nchild == 0) P2M(c);
+      for (int d = (int)lv.size() - 1; d >= 0; d--) {
+        const std::vector<int> &cs = lv[d];
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < (int)cs.size(); i++)
+          if (cells[cs[i]].nchild != 0) M2M(cs[i]);
+      }
+    }
     for (int c = nc - 1; c >= 0; c--) {
       OCell &C = cells[c];
-      if (C.nchild == 0)
-        P2M(c);
-      else
-        M2M(c);
+      if (!parallel()) {
+        if (C.nchild == 0)
+          P2M(c);
+        else
+          M2M(c);
+      }
       if (p.useRmax) {  // setRmax, :51-67
         double Rmax = 0;
         if (C.nchild == 0) {
@@ -984,7 +1010,10 @@ struct orc_ctx {
       m2l_j.push_back(ci);
       m2l_img.push_back(img_cur);
     }
-    M2L(ci, cj, Xp_cur, mutual);
+    if (deferring)
+      defer(ci, cj, 0);
+    else
+      M2L(ci, cj, Xp_cur, mutual);
   }
   void emit_p2p(int ci, int cj, bool self, bool mutual) {
     p2p_i.push_back(ci);
@@ -995,10 +1024,43 @@ struct orc_ctx {
       p2p_j.push_back(ci);
       p2p_img.push_back(img_cur);
     }
-    if (self)
+    if (deferring)
+      defer(ci, cj, self ? 2 : 1);
+    else if (self)
       P2P_self(ci);
     else
       P2P(ci, cj, Xp_cur, mutual);
+  }
+  void defer(int ci, int cj, int kind) {
+    def_ci.push_back(ci);
+    def_cj.push_back(cj);
+    def_kind.push_back((unsigned char)kind);
+  }
+  // The recorded calls grouped by target (stable, so emission order within
+  // a target), targets in parallel.
+  void run_deferred() {
+    const int nc = (int)cells.size();
+    const size_t nd = def_ci.size();
+    std::vector<size_t> off(nc + 1, 0), order(nd);
+    for (size_t e = 0; e < nd; e++) off[def_ci[e] + 1]++;
+    for (int c = 0; c < nc; c++) off[c + 1] += off[c];
+    std::vector<size_t> fill(off.begin(), off.end() - 1);
+    for (size_t e = 0; e < nd; e++) order[fill[def_ci[e]]++] = e;
+    const double zero[3] = {0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int c = 0; c < nc; c++)
+      for (size_t k = off[c]; k < off[c + 1]; k++) {
+        const size_t e = order[k];
+        if (def_kind[e] == 0)
+          M2L(c, def_cj[e], zero, false);
+        else if (def_kind[e] == 1)
+          P2P(c, def_cj[e], zero, false);
+        else
+          P2P_self(c);
+      }
+    def_ci.clear();
+    def_cj.clear();
+    def_kind.clear();
   }
 
   void traverse(int ci, int cj, bool mutual) {
@@ -1176,7 +1238,10 @@ struct orc_ctx {
     if (p.images == 0) {
       for (int d = 0; d < 3; d++) Xp_cur[d] = 0;
       img_cur = 13;
+      deferring = parallel() && p.mutual == 0;
       traverse(0, 0, p.mutual != 0);
+      if (deferring) run_deferred();
+      deferring = false;
     } else {
       for (int ix = -1; ix <= 1; ix++)
         for (int iy = -1; iy <= 1; iy++)
@@ -1326,8 +1391,37 @@ struct orc_ctx {
     }
   }
 
+  // Cells grouped by depth (root 0); parents precede children in both
+  // layouts, so one forward sweep finds every depth.
+  std::vector<std::vector<int>> cells_by_depth() const {
+    const int nc = (int)cells.size();
+    std::vector<int> depth(nc, 0);
+    std::vector<std::vector<int>> lv;
+    for (int c = 0; c < nc; c++) {
+      if (c > 0) depth[c] = depth[cells[c].iparent] + 1;
+      if ((int)lv.size() <= depth[c]) lv.resize(depth[c] + 1);
+      lv[depth[c]].push_back(c);
+    }
+    return lv;
+  }
+
   void downward() {
     if (cells.empty()) return;
+    if (parallel()) {
+      // Depth by depth: L2L reads the parent's finished L and writes the
+      // cell's own, L2P writes only the cell's bodies.
+      std::vector<std::vector<int>> lv = cells_by_depth();
+      if (cells[0].nchild == 0) L2P(0);
+      for (size_t d = 1; d < lv.size(); d++) {
+        const std::vector<int> &cs = lv[d];
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int i = 0; i < (int)cs.size(); i++) {
+          L2L(cs[i]);
+          if (cells[cs[i]].nchild == 0) L2P(cs[i]);
+        }
+      }
+      return;
+    }
     if (cells[0].nchild == 0) L2P(0);
     // Parents precede children in both layouts: a forward sweep is pre-order.
     for (int c = 1; c < (int)cells.size(); c++) {
@@ -1373,6 +1467,9 @@ int orc_set_bodies(orc_ctx *c, int n, const double *xyzq) {
 }
 
 int orc_run(orc_ctx *c, int stages) {
+  // threads: 1 = the serial recursion; > 1 that many OpenMP threads; 0 the
+  // OpenMP default.
+  if (c->p.threads > 1) omp_set_num_threads(c->p.threads);
   if (stages & 1) c->bounds();
   if (stages & 2) {
     if (c->p.adaptive)

@@ -97,3 +97,34 @@ def test_batch_rejects_unsupported():
     with pytest.raises(X.ExafmmError):
         f.evaluate_batch([np.zeros((0, 4))])
     f.close()
+
+
+def test_autotune_picks_parameters_and_matches_untuned_run():
+    """With autotuning on, the first batch of a signature times candidate
+    configurations and keeps the fastest; the results equal an untuned run at
+    the chosen ncrit bit for bit (the P2P and contraction grids do not change
+    any sum), and a second call reuses the choice."""
+    sy = systems([3000, 20000, 45000, 9000, 70000], seed=5)
+    print("[gpu] batch autotune", flush=True)
+    f = X.FMM(P=8, theta=0.5, ncrit=64)
+    f.set_autotune(True)
+    a = f.evaluate_batch(sy)
+    t = f.tuning()
+    assert t["ncrit"] in (32, 64, 128) and t["p2p_waves_per_cu"] in (8, 12, 16)
+    assert t["p2p_targets_per_lane"] in (3, 4, 5) and t["m2l_groups_per_cu"] in (0, 2)
+    b = f.evaluate_batch(sy)
+    assert f.tuning() == t
+    g = X.FMM(P=8, theta=0.5, ncrit=t["ncrit"])
+    c = g.evaluate_batch(sy)
+    for u, v, w in zip(a, b, c):
+        assert np.array_equal(u, w) and np.array_equal(v, w)
+    # A single system of its own signature.
+    f.set_bodies(sy[2])
+    f.evaluate()
+    ts = f.tuning()
+    g2 = X.FMM(P=8, theta=0.5, ncrit=ts["ncrit"])
+    g2.set_bodies(sy[2])
+    g2.evaluate()
+    assert np.array_equal(f.results(), g2.results())
+    for h in (f, g, g2):
+        h.close()

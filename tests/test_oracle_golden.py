@@ -83,3 +83,21 @@ def test_oracle_direct_matches_reference(name):
     t = x[z["sample_ibody"]]
     d = O.direct(t, x, images=int(z["images"]), cycle=2 * np.pi)
     assert rel_l2(d, z["sample_direct"]) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["cube16k_u_p10", "plummer16k_a_p10", "mass_plummer16k_a_p10", "cube16k_u_p6_cart",
+                                  "cube4k_u_p10_mut", "cube2k_u_p6_periodic"])
+def test_parallel_oracle_is_bit_identical_to_serial(name):
+    """The oracle's parallel mode (target cells' kernel calls deferred and run
+    in emission order, passes depth by depth) against its serial recursion:
+    every expansion and result bit for bit."""
+    z = load_case(name)
+    out = []
+    for threads in (1, 0):
+        o = O.Oracle(**dict(case_params(z), threads=threads))
+        o.set_bodies(z["input"])
+        o.run()
+        c, b = o.cells(), o.bodies()
+        out.append((c["M"], c["L"], b["trg"]))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
