@@ -28,7 +28,15 @@ import exafmm_hip  # noqa: E402
 
 METRIC = "FMM particles/s at N=1M P=10, mixed vs FP64 GPU; P2P Ginter/s; rel L2 err"
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and MFMA)
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 (vector and MFMA)
 PEAK_HBM_GBS = 8000.0
+
+
+def p2p_kernel_name(tuning, form):
+    """Kernel name of the P2P variant the library ran (get_tuning:
+    p2p_targets_per_lane = 1 for the one-target-per-lane kernel)."""
+    k = tuning["p2p_targets_per_lane"]
+    return f"k_p2p_{form}" if k == 1 else f"k_p2p_{form}_rb<{k}>"
 
 
 def m2l_flops_per_pair(P):
@@ -368,6 +376,7 @@ def main():
         f.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         stats[-1]["num_p2p_bodies"] = f.count_p2p_bodies()  # outside the timed steps
+        tuning = f.tuning()
         # Accuracy against FP64 direct summation on 128 random targets.
         res = f.results()[tgt]
         ref = f.direct(tgt)
@@ -375,39 +384,62 @@ def main():
         err_acc = float(np.sqrt(np.sum((res[:, 1:] - ref[:, 1:]) ** 2) / np.sum(ref[:, 1:] ** 2)))
         f.close()
         med = {k: float(np.median([d[k] for d in stats])) for k in stats[0] if k.startswith("t_")}
-        return ms, stats[-1], med, err_pot, err_acc
+        return ms, stats[-1], med, err_pot, err_acc, tuning
 
-    ms, s, med, err_pot, err_acc = run("fp64")
+    ms, s, med, err_pot, err_acc, fp64_tuning = run("fp64")
     value = n / (ms * 1e-3)
     mixed = None
+    mixed_med, mixed_tuning = None, None
     if args.precision == "mixed" or args.with_mixed:
-        mms, ms_stats, mmed, mpot, macc = run("mixed")
+        mms, ms_stats, mmed, mpot, macc, mixed_tuning = run("mixed")
+        mixed_med = mmed
         mixed = {"value": n / (mms * 1e-3), "unit": "particles/s", "ms_per_step": mms,
                  "speedup_vs_fp64": ms / mms, "rel_l2_err": {"pot": mpot, "acc": macc, "targets": 128},
                  "phases_ms": {k: round(v, 4) for k, v in mmed.items()},
                  "dtype": "f32 P2P (compensated) and f32 M2L tables; f64 centres, offsets, expansions"}
 
     # Per-kernel rates (algorithmic work / HIP-event time of that kernel).
+    # P2P and the per-pair M2L run on the vector ALU, the lattice M2L on the
+    # matrix cores; on MI355X the FP64 vector and matrix peaks are equal.
     t_m2l, t_p2p = med["t_m2l"], med["t_p2p"]
     t_mfma, t_pairs = med["t_m2l_mfma"], med["t_m2l_pairs"]
     n_lat = s["num_m2l_lattice"]
     n_irr = s["num_m2l_pairs"] - n_lat
     fpp = m2l_flops_per_pair(P)
     p2p_pairs = s["num_p2p_bodies"]
+    p2p_name = p2p_kernel_name(fp64_tuning, "f64")
     kern = {
         "k_m2l_mfma<%d,double>" % P: (n_lat * fpp, t_mfma, "mfma", PEAK_FP64_TFLOPS),
-        "k_m2l<%d,double>" % P: (n_irr * fpp, t_pairs, "mfma", PEAK_FP64_TFLOPS),
-        "k_p2p_f64": (20 * p2p_pairs, t_p2p, "mfma", PEAK_FP64_TFLOPS),
+        "k_m2l<%d,double>" % P: (n_irr * fpp, t_pairs, "valu", PEAK_FP64_TFLOPS),
+        p2p_name: (20 * p2p_pairs, t_p2p, "valu", PEAK_FP64_TFLOPS),
     }
     kernels_tflops = {k: round(w / (t * 1e-3) / 1e12, 4) for k, (w, t, _, _) in kern.items() if t > 0}
     dom_name = max((k for k in kern if kern[k][1] > 0), key=lambda k: kern[k][1])
+
+    def rate(name, work, t, bound, peak):
+        a = work / (t * 1e-3) / 1e12
+        return {"bound": bound, "achieved": round(a, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 5),
+                "kernel_ms": round(t, 4)}
+
     w, t, bound, peak = kern[dom_name]
-    ach = w / (t * 1e-3) / 1e12
-    roofline = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 5), "traffic": measured_traffic(dom_name) if n == 1 << 20 and P == 10 else None,
-                "kernel": dom_name, "kernel_ms": round(t, 4),
-                "note": ("kernel duration in the timed steps, where the contraction and P2P share the CUs "
-                         "(P2P: FP64 vector ALU, its peak equals the FP64 matrix peak)")}
+    roofline = rate(dom_name, w, t, bound, peak)
+    roofline.update({"traffic": measured_traffic(dom_name) if n == 1 << 20 and P == 10 else None,
+                     "kernel": dom_name,
+                     "note": ("dominant kernel = the longest HIP-event duration in the timed steps, where the "
+                              "contraction and P2P share the CUs; P2P runs on the FP64 vector ALU "
+                              "(bound 'valu'), whose peak equals the FP64 matrix peak")})
+    # Both heavy kernels of each form, in the timed steps.
+    roofline["kernels"] = {k: dict(rate(k, *kern[k]), traffic=measured_traffic(k) if n == 1 << 20 and P == 10 else None)
+                           for k in (p2p_name, "k_m2l_mfma<%d,double>" % P) if kern[k][1] > 0}
+    if mixed is not None:
+        mp2p = p2p_kernel_name(mixed_tuning, "mixed")
+        # FP32 arithmetic: against the published FP32 vector / matrix peak.
+        for k, work, tk, bound, peak in ((mp2p, 20 * p2p_pairs, mixed_med["t_p2p"], "valu", PEAK_FP32_TFLOPS),
+                                         ("k_m2l_mfma<%d,float>" % P, n_lat * fpp, mixed_med["t_m2l_mfma"], "mfma",
+                                          PEAK_FP32_TFLOPS)):
+            if tk > 0:
+                roofline["kernels"][k] = dict(rate(k, work, tk, bound, peak),
+                                              traffic=measured_traffic(k) if n == 1 << 20 and P == 10 else None)
     # The same two kernels with the CUs to themselves: P2P as one wave per
     # leaf (EXAFMM_HIP_P2P_WAVES=0) fills the GPU, so the contraction starts
     # after it; 3 evaluations after the timed ones, outside the timed region.
@@ -430,7 +462,7 @@ def main():
             else:
                 os.environ["EXAFMM_HIP_P2P_WAVES"] = old_env
         iso_t = {"k_m2l_mfma<%d,double>" % P: float(np.median([d["t_m2l_mfma"] for d in iso])),
-                 "k_p2p_f64": float(np.median([d["t_p2p"] for d in iso]))}
+                 p2p_name: float(np.median([d["t_p2p"] for d in iso]))}
         roofline["isolated"] = {}
         for k, ti in iso_t.items():
             if ti > 0:

@@ -150,8 +150,10 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
   if (const char *m = std::getenv("EXAFMM_HIP_M2L")) H->h.lattice_m2l = std::strcmp(m, "pairs") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_UPDOWN")) H->h.updown_mfma = std::strcmp(m, "cells") != 0;
   if (const char *m = std::getenv("EXAFMM_HIP_P2P_WAVES")) H->h.p2p_wpc = std::atoi(m);  // per CU
-  if (const char *m = std::getenv("EXAFMM_HIP_P2P_KERNEL")) H->h.p2p_kernel = std::atoi(m);
-  if (const char *m = std::getenv("EXAFMM_HIP_P2P_KMAX")) H->h.p2p_kmax = std::atoi(m);
+  if (const char *m = std::getenv("EXAFMM_HIP_P2P_KMAX")) {  // P2P targets per lane: 1, 3, 4 or 5
+    const int k = std::atoi(m);
+    if (k == 1 || (k >= 3 && k <= 5)) H->h.p2p_kmax = k;
+  }
   if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
   // Rows of the lattice T(d) table kept across evaluations (16384 rows are
   // 0.5 GB in FP64 at P = 10).
@@ -355,9 +357,9 @@ static void apply_tuned(Handle &h, const Handle::Tuned &t) {
 // One evaluation with the tuned parameters of the input's signature; an
 // unseen signature is tuned first by timing candidate evaluations (device
 // time, best of two each): ncrit in {32, 64, 128}, then the P2P grid (8, 12,
-// 16 waves per CU), the P2P targets per lane (3, 4, 5) and the contraction
-// grid (one workgroup per item, or 2 per CU looping over the items), each
-// step keeping the best so far.
+// 16 waves per CU), the P2P targets per lane (1, 3, 4, 5) and the
+// contraction grid (one workgroup per item, or 2 per CU looping over the
+// items), each step keeping the best so far.
 static void evaluate_tuned(Handle &h) {
   if (!h.autotune) return evaluate_all(h);
   const std::vector<int64_t> key = tune_signature(h);
@@ -380,7 +382,8 @@ static void evaluate_tuned(Handle &h) {
     const Handle::Tuned base = best;
     for (int w : {8, 16}) trial({base.ncrit, w, base.p2p_kmax, base.lat_wg_cu, 0});
     const Handle::Tuned b2 = best;
-    for (int k : {3, 5}) trial({b2.ncrit, b2.p2p_wpc, k, b2.lat_wg_cu, 0});
+    for (int k : {1, 3, 4, 5})
+      if (k != b2.p2p_kmax) trial({b2.ncrit, b2.p2p_wpc, k, b2.lat_wg_cu, 0});
     const Handle::Tuned b3 = best;
     trial({b3.ncrit, b3.p2p_wpc, b3.p2p_kmax, b3.lat_wg_cu == 0 ? 2 : 0, 0});
     it = h.tune_cache.emplace(key, best).first;
@@ -404,6 +407,16 @@ int exafmm_hip_get_tuning(exafmm_hip_handle *H, int *ncrit, int *p2p_waves_per_c
     if (p2p_waves_per_cu) *p2p_waves_per_cu = h.p2p_wpc;
     if (p2p_targets_per_lane) *p2p_targets_per_lane = h.p2p_kmax;
     if (m2l_groups_per_cu) *m2l_groups_per_cu = h.lat_wg_cu;
+  });
+}
+
+int exafmm_hip_set_tuning(exafmm_hip_handle *H, int ncrit, int p2p_waves_per_cu, int p2p_targets_per_lane,
+                          int m2l_groups_per_cu) {
+  return guarded(H, [&](Handle &h) {
+    ensure(ncrit > 0 && p2p_waves_per_cu >= 0 && m2l_groups_per_cu >= 0 &&
+               (p2p_targets_per_lane == 1 || (p2p_targets_per_lane >= 3 && p2p_targets_per_lane <= 5)),
+           "set_tuning: bad parameters");
+    apply_tuned(h, Handle::Tuned{ncrit, p2p_waves_per_cu, p2p_targets_per_lane, m2l_groups_per_cu, 0});
   });
 }
 

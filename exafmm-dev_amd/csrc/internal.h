@@ -147,8 +147,10 @@ struct Handle {
   // workgroups per CU looping over (tile, segment) items (0: one workgroup
   // per item).  ncu = compute units.
   int p2p_wpc = 12, lat_wg_cu = 0, ncu = 0;
-  int p2p_kernel = 1;  // P2P: 1 one target per lane (k_p2p_f64 / k_p2p_mixed), 0 several (the _rb kernels)
-  int p2p_kmax = 4;    // k_p2p_f64_rb: the most targets per lane (3, 4 or 5)
+  // P2P targets per lane: 1 = one target per lane (k_p2p_f64 / k_p2p_mixed),
+  // 3, 4 or 5 = the most targets per lane of the register-blocked kernels
+  // (k_p2p_f64_rb / k_p2p_mixed_rb).
+  int p2p_kmax = 1;
   DBuf<double> ud_ops;
   DBuf<int> lat_tkey, lat_tval, lat_thist, lat_gb, lat_act, lat_aidx, lat_alist, lat_step0;
   DBuf<double> lat_part;

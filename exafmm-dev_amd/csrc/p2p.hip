@@ -473,6 +473,16 @@ __global__ void __launch_bounds__(64, 4)
 // accumulator: the per-target compensated summation of the reference's
 // kahan<T> (kahan.h:35-41), applied per run instead of per term.  The
 // compensated value is read out as s - c.
+//
+// The lo parts matter only for close pairs.  A target lies inside its leaf
+// (|x_i - X| <= sqrt(3) w / 2 for leaf width w = 2 Rb), so a source within r
+// of it has |x_j - X| <= sqrt(3) w / 2 + r, and the hi-only difference errs
+// by at most 2^-24 (sqrt(3) w + r) per component: a relative error below
+// 2^-24 (sqrt(3) w / r + 1), i.e. 1.7e-6, for r >= w / 16.  Stages that
+// cannot hold the target's own position therefore take the hi parts alone
+// and add the lo parts only in the (wave-uniform, rare) case that a lane's
+// hi-only R2 falls below (w / 16)^2 = Rb^2 / 64; stages that may hold the
+// target itself (its own leaf, periodic images) always take both.
 constexpr int kP2PStageF = 256;
 static_assert(kP2PStageF % 64 == 0, "stage tiles are padded to multiples of 64");
 constexpr int kSeg = 8;  // FP32 pair terms summed before each compensated add
@@ -494,8 +504,8 @@ __global__ void __launch_bounds__(64)
     k_p2p_mixed(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
                 int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
                 const unsigned char *__restrict__ img, const int *__restrict__ cBody,
-                const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
-                double4 *__restrict__ trg, double cycle) {
+                const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double *__restrict__ cRb,
+                const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
   __shared__ __attribute__((aligned(32))) float4 sb[kP2PStageF];  // hi parts and charge
   __shared__ float4 sbl[kP2PStageF];                               // lo parts
   double4 *const sacc = reinterpret_cast<double4 *>(sb);           // (as in k_p2p_f64)
@@ -509,6 +519,7 @@ __global__ void __launch_bounds__(64)
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
     const double4 Xc = cX[c];
+    const float thr2 = (float)(cRb[c] * cRb[c] * (1.0 / 64));  // (w / 16)^2: pairs that need the lo parts
     const int e0 = off[c], e1 = off[c + 1];
     // A target's position relative to the leaf centre, FP32 hi + lo.
     struct Tgt {
@@ -541,19 +552,30 @@ __global__ void __launch_bounds__(64)
       // step of the pair arithmetic for all of them), summed in source order.
 #pragma unroll
       for (int u0 = 0; u0 < kSeg; u0 += kSegRun) {
-        float4 xj[kSegRun], xl[kSegRun];
+        float4 xj[kSegRun];
         float dx[kSegRun], dy[kSegRun], dz[kSegRun], r2[kSegRun], rs[kSegRun];
 #pragma unroll
-        for (int u = 0; u < kSegRun; u++) {
-          xj[u] = sg[(u0 + u) * G];
-          xl[u] = sgl[(u0 + u) * G];
-        }
+        for (int u = 0; u < kSegRun; u++) xj[u] = sg[(u0 + u) * G];
+        bool close = decltype(mask)::value;
 #pragma unroll
         for (int u = 0; u < kSegRun; u++) {
-          dx[u] = (t.x - xj[u].x) + (t.xl - xl[u].x);
-          dy[u] = (t.y - xj[u].y) + (t.yl - xl[u].y);
-          dz[u] = (t.z - xj[u].z) + (t.zl - xl[u].z);
+          dx[u] = t.x - xj[u].x;
+          dy[u] = t.y - xj[u].y;
+          dz[u] = t.z - xj[u].z;
           r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          if (!decltype(mask)::value) close |= r2[u] < thr2;
+        }
+        // The lo parts: always where the stage may hold the target itself,
+        // else only if some lane has a close pair (uniform branch).
+        if (decltype(mask)::value || __any(close)) {
+#pragma unroll
+          for (int u = 0; u < kSegRun; u++) {
+            const float4 xl = sgl[(u0 + u) * G];
+            dx[u] = dx[u] + (t.xl - xl.x);
+            dy[u] = dy[u] + (t.yl - xl.y);
+            dz[u] = dz[u] + (t.zl - xl.z);
+            r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          }
         }
 #pragma unroll
         for (int u = 0; u < kSegRun; u++) rs[u] = __builtin_amdgcn_rsqf(r2[u]);
@@ -670,41 +692,58 @@ struct RbTgt32 {
   float x, y, z, xl, yl, zl;
 };
 
-template <bool MASK>
-__device__ inline void rb_pair32(const RbTgt32 &t, const float4 &s, const float4 &sl, float4 &a) {
-  const float dx = (t.x - s.x) + (t.xl - sl.x);
-  const float dy = (t.y - s.y) + (t.yl - sl.y);
-  const float dz = (t.z - s.z) + (t.zl - sl.z);
-  const float r2 = dx * dx + dy * dy + dz * dz;
-  const float rs = __builtin_amdgcn_rsqf(r2);
-  const float invr = MASK ? (r2 > 0.0f ? rs : 0.0f) : rs;
-  const float qinvr = s.w * invr;
-  const float qinvr3 = qinvr * invr * invr;
-  a.x += qinvr;
-  a.y += dx * qinvr3;
-  a.z += dy * qinvr3;
-  a.w += dz * qinvr3;
+// The pair terms of K targets against one source s (hi parts and charge);
+// sl: the source's lo parts, read only where needed (see k_p2p_mixed).
+template <bool MASK, int K>
+__device__ inline void rb_pairs32(const RbTgt32 (&t)[K], const float4 &s, const float4 *__restrict__ sl, float thr2,
+                                  float4 (&a)[K]) {
+  float dx[K], dy[K], dz[K], r2[K];
+  bool close = MASK;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    dx[k] = t[k].x - s.x;
+    dy[k] = t[k].y - s.y;
+    dz[k] = t[k].z - s.z;
+    r2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+    if (!MASK) close |= r2[k] < thr2;
+  }
+  if (MASK || __any(close)) {
+    const float4 l = *sl;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      dx[k] = dx[k] + (t[k].xl - l.x);
+      dy[k] = dy[k] + (t[k].yl - l.y);
+      dz[k] = dz[k] + (t[k].zl - l.z);
+      r2[k] = dx[k] * dx[k] + dy[k] * dy[k] + dz[k] * dz[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const float rs = __builtin_amdgcn_rsqf(r2[k]);
+    const float invr = MASK ? (r2[k] > 0.0f ? rs : 0.0f) : rs;
+    const float qinvr = s.w * invr;
+    const float qinvr3 = qinvr * invr * invr;
+    a[k].x += qinvr;
+    a[k].y += dx[k] * qinvr3;
+    a[k].z += dy[k] * qinvr3;
+    a[k].w += dz[k] * qinvr3;
+  }
 }
 
-// One tile: sources sg[0], sg[8], ... (ncp a multiple of 8 kSeg); each run
-// of kSeg sources of a lane is summed in FP32 per target, then added into
-// the target's compensated sums.
 template <bool MASK, int K>
 __device__ inline void rb_tile32(const float4 *__restrict__ sg, const float4 *__restrict__ sgl, int ncp,
-                                 const RbTgt32 (&t)[K], Kahan32 (&kp)[K][4]) {
-  float4 cur = sg[0], curl = sgl[0];
+                                 const RbTgt32 (&t)[K], float thr2, Kahan32 (&kp)[K][4]) {
+  float4 cur = sg[0];
   for (int j = 0; j < ncp; j += kRbG * kSeg) {
     float4 run[K];
 #pragma unroll
     for (int k = 0; k < K; k++) run[k] = make_float4(0, 0, 0, 0);
 #pragma unroll
     for (int u = 0; u < kSeg; u++) {
-      const int jn = j + (u + 1) * kRbG;
-      const float4 nxt = sg[jn], nxtl = sgl[jn];
-#pragma unroll
-      for (int k = 0; k < K; k++) rb_pair32<MASK>(t[k], cur, curl, run[k]);
+      const int jc = j + u * kRbG;
+      const float4 nxt = sg[jc + kRbG];
+      rb_pairs32<MASK, K>(t, cur, sgl + jc, thr2, run);
       cur = nxt;
-      curl = nxtl;
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -719,8 +758,9 @@ __device__ inline void rb_tile32(const float4 *__restrict__ sg, const float4 *__
 template <int K>
 __device__ void rb_block32(int lane, int c, int tb0, int tbase, int nt, int e0, int e1, const int *__restrict__ src,
                            const unsigned char *__restrict__ img, const int *__restrict__ cBody,
-                           const int *__restrict__ cNBody, const double4 &Xc, const double4 *__restrict__ body,
-                           double4 *__restrict__ trg, double cycle, int tn, float4 *sb, float4 *sbl, SrcDesc &sd) {
+                           const int *__restrict__ cNBody, const double4 &Xc, float thr2,
+                           const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle, int tn, float4 *sb,
+                           float4 *sbl, SrcDesc &sd) {
   const int g = lane / kRbG, i = lane % kRbG;
   RbTgt32 t[K];
   Kahan32 kp[K][4];
@@ -755,9 +795,9 @@ __device__ void rb_block32(int lane, int c, int tb0, int tbase, int nt, int e0, 
     __syncthreads();
     const int ncp = __builtin_amdgcn_readfirstlane(cntp);
     if (masked)
-      rb_tile32<true, K>(sb + g, sbl + g, ncp, t, kp);
+      rb_tile32<true, K>(sb + g, sbl + g, ncp, t, thr2, kp);
     else
-      rb_tile32<false, K>(sb + g, sbl + g, ncp, t, kp);
+      rb_tile32<false, K>(sb + g, sbl + g, ncp, t, thr2, kp);
   };
   for (int s0 = 0; s0 < tn; s0 += kRbStage) {
     const int cnt = min(kRbStage, tn - s0);
@@ -810,8 +850,8 @@ __global__ void __launch_bounds__(64, 4)
     k_p2p_mixed_rb(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
                    int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
                    const unsigned char *__restrict__ img, const int *__restrict__ cBody,
-                   const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double4 *__restrict__ body,
-                   double4 *__restrict__ trg, double cycle) {
+                   const int *__restrict__ cNBody, const double4 *__restrict__ cX, const double *__restrict__ cRb,
+                   const double4 *__restrict__ body, double4 *__restrict__ trg, double cycle) {
   // Stage (hi parts and charge, then lo parts), padded to whole runs plus
   // the read-ahead; the block's partials (64 double4) reuse the hi array.
   constexpr int kCap = kRbStage + kRbG * kSeg + kRbG;
@@ -824,12 +864,13 @@ __global__ void __launch_bounds__(64, 4)
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
     const double4 Xc = cX[c];
+    const float thr2 = (float)(cRb[c] * cRb[c] * (1.0 / 64));  // as in k_p2p_mixed
     const int e0 = off[c], e1 = off[c + 1];
     for (int tbase = 0; tbase < tn; tbase += kRbG * KMAX) {
       const int nt = min(kRbG * KMAX, tn - tbase);
       const int K = (nt + kRbG - 1) / kRbG;
 #define EXA_RB_BLOCK32(k) \
-  rb_block32<k>(lane, c, tb0, tbase, nt, e0, e1, src, img, cBody, cNBody, Xc, body, trg, cycle, tn, sb, sbl, sd)
+  rb_block32<k>(lane, c, tb0, tbase, nt, e0, e1, src, img, cBody, cNBody, Xc, thr2, body, trg, cycle, tn, sb, sbl, sd)
       if (K <= 1)
         EXA_RB_BLOCK32(1);
       else if (K == 2)
@@ -878,16 +919,17 @@ void run_p2p(Handle &h, hipStream_t st) {
   }
   h.p2p_next.reserve(1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, h.p2p_next.p);
-  if (h.cfg.precision == 1 && h.p2p_kernel == 1)
+  if (h.cfg.precision == 1 && h.p2p_kmax == 1)
     hipLaunchKernelGGL(k_p2p_mixed, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
-                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p, h.cfg.cycle);
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.cRb.p, h.body4.p,
+                       h.trg4.p, h.cfg.cycle);
   else if (h.cfg.precision == 1) {
     auto *kern = h.p2p_kmax == 3 ? k_p2p_mixed_rb<3> : h.p2p_kmax == 5 ? k_p2p_mixed_rb<5> : k_p2p_mixed_rb<4>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
-                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.body4.p, h.trg4.p,
-                       h.cfg.cycle);
+                       h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.cX.p, h.cRb.p, h.body4.p,
+                       h.trg4.p, h.cfg.cycle);
   }
-  else if (h.p2p_kernel == 1)
+  else if (h.p2p_kmax == 1)
     hipLaunchKernelGGL(k_p2p_f64, dim3(grid), dim3(64), 0, st, h.p2p_tgt.p, h.np2p_tgt, ntl_dev, h.p2p_next.p,
                        h.p2p_off.p, h.p2p_src.p, h.p2p_img.p, h.cBody.p, h.cNBody.p, h.body4.p, h.trg4.p, h.cfg.cycle);
   else {

@@ -27,7 +27,7 @@ EXPORTS = (
     "exafmm_hip_evaluate_batch", "exafmm_hip_direct_arrays_static",
     "exafmm_hip_cell_p2p", "exafmm_hip_cell_p2p_self", "exafmm_hip_cell_p2m", "exafmm_hip_cell_m2m",
     "exafmm_hip_cell_m2l", "exafmm_hip_cell_l2l", "exafmm_hip_cell_l2p",
-    "exafmm_hip_set_autotune", "exafmm_hip_get_tuning",
+    "exafmm_hip_set_autotune", "exafmm_hip_get_tuning", "exafmm_hip_set_tuning",
 )
 
 
@@ -108,6 +108,7 @@ def load():
     L.exafmm_hip_cell_l2p.argtypes = [vp, ip, vp, vp, vp, vp]
     L.exafmm_hip_set_autotune.argtypes = [vp, ip]
     L.exafmm_hip_get_tuning.argtypes = [vp, P(ip), P(ip), P(ip), P(ip)]
+    L.exafmm_hip_set_tuning.argtypes = [vp, ip, ip, ip, ip]
     L.exafmm_hip_set_config.argtypes = [vp, P(Config)]
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]
@@ -331,6 +332,12 @@ class FMM:
         self._check(self.lib.exafmm_hip_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "get_tuning")
         return dict(zip(("ncrit", "p2p_waves_per_cu", "p2p_targets_per_lane", "m2l_groups_per_cu"),
                         (x.value for x in v)))
+
+    def set_tuning(self, ncrit, p2p_waves_per_cu, p2p_targets_per_lane, m2l_groups_per_cu):
+        """Pin the parameters that tuning() reports (e.g. the ones a tuned
+        run chose)."""
+        self._check(self.lib.exafmm_hip_set_tuning(self.h, int(ncrit), int(p2p_waves_per_cu),
+                                                   int(p2p_targets_per_lane), int(m2l_groups_per_cu)), "set_tuning")
 
     def count_p2p_bodies(self):
         """Target-source body pairs of the current P2P list."""

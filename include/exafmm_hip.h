@@ -194,8 +194,9 @@ int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
  * signature (the settings that change the work, the system count, the
  * systems' sizes by powers of two and the total size to ~6%): the first call
  * with a new signature times candidate evaluations -- ncrit in {32, 64, 128},
- * P2P grid of 8, 12 or 16 waves per CU, 3, 4 or 5 P2P targets per lane, and
- * the contraction grid -- and later calls reuse the winner.  ncrit is part of
+ * P2P grid of 8, 12 or 16 waves per CU, 1, 3, 4 or 5 P2P targets per lane
+ * (1: one target per lane; 3-5: the register-blocked P2P kernels), and the
+ * contraction grid -- and later calls reuse the winner.  ncrit is part of
  * the FMM (it changes the tree and so the approximation), so a tuned run is
  * a valid FMM evaluation at the chosen ncrit, not bit-identical to another.
  * exafmm_hip_get_tuning reports the parameters in use (any pointer may be
@@ -203,6 +204,13 @@ int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 int exafmm_hip_set_autotune(exafmm_hip_handle *h, int enable);
 int exafmm_hip_get_tuning(exafmm_hip_handle *h, int *ncrit, int *p2p_waves_per_cu, int *p2p_targets_per_lane,
                           int *m2l_groups_per_cu);
+/* Set those parameters directly (as a tuned run would; a changed ncrit
+ * invalidates the tree): ncrit > 0, P2P waves per CU >= 0 (0: one wave per
+ * target leaf), P2P targets per lane 1, 3, 4 or 5, contraction workgroups per
+ * CU >= 0 (0: one per work item).  With autotuning on, the next evaluation of
+ * an already tuned signature replaces them with its tuned values. */
+int exafmm_hip_set_tuning(exafmm_hip_handle *h, int ncrit, int p2p_waves_per_cu, int p2p_targets_per_lane,
+                          int m2l_groups_per_cu);
 
 /* Target-source body pairs of the current P2P list (a reduction on the
  * device; also stored into the stats' num_p2p_bodies until the next
