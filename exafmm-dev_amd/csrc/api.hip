@@ -99,7 +99,9 @@ static void do_traverse(Handle &h) {
   EXA_HIP(hipEventRecord(h.ev[5], h.stream));
   EXA_HIP(hipStreamWaitEvent(h.stream2, h.ev[5], 0));
   // The M2L set-up's first kernels are queued before P2P: they run while the
-  // P2P list is grouped, before P2P fills the GPU.
+  // P2P list is grouped, before P2P fills the GPU.  (P2P queued first was
+  // measured no faster: the set-up kernels then share the CUs with P2P's
+  // waves and take several times longer.)
   EXA_HIP(hipEventRecord(h.ev[6], h.stream));
   m2l_begin(h);
   EXA_HIP(hipEventRecord(h.ev[9], h.stream2));

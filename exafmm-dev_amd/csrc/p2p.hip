@@ -175,10 +175,6 @@ __global__ void __launch_bounds__(64)
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
     const int e0 = off[c], e1 = off[c + 1];
-    // MASK: the stage may hold a body at the target's own position (its own
-    // leaf, or a periodic image), so R2 == 0 is skipped by a select; other
-    // stages have R2 > 0 for every pair (equal positions share a Morton key
-    // and hence a leaf).
     // Sum the G group partials of the pass's targets in group order.
     auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
       __syncthreads();
@@ -209,84 +205,102 @@ __global__ void __launch_bounds__(64)
       const double4 xa = ia < nt ? body[tb0 + tbase + ia] : make_double4(0, 0, 0, 0);
       const double4 xb = ib < tp.r ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
       double4 acca = make_double4(0, 0, 0, 0), accb = make_double4(0, 0, 0, 0);
-      SrcWalker w{e0, 0, e0, 0};
-      while (w.e < e1) {
-        // Fill: consecutive source leaves (or parts of one) up to kP2PStage
-        // bodies, each shifted by its periodic image.
-        bool own = false;
-        const int cnt = fill_stage<kP2PStage>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
-          double px, py, pz;
-          image_shift(im, cycle, px, py, pz);
-          double4 v = body[b];
-          v.x += px;
-          v.y += py;
-          v.z += pz;
-          sb[k] = v;
-          own |= (b >= tb0 && b < tb0 + tn) || im != 13;
-        });
-        // Pad the stage to a multiple of kP2PUnroll G (G of either pass, at
-        // most 8; the stage size is a multiple of 64) with zero charges far
-        // away, which add exactly zero, so both passes run whole groups of
-        // kP2PUnroll sources.
+      // One pass over a staged tile with G (1, 2, 4 or 8) a compile-time
+      // constant and a wave-uniform trip count: the source addresses of a
+      // group are one base and immediate offsets.  MASK: the tile may hold a
+      // body at the target's own position (its own leaf, or a periodic
+      // image), so R2 == 0 is skipped by a select; other tiles have R2 > 0
+      // for every pair (equal positions share a Morton key and hence a leaf).
+      auto pass_g = [&](auto mask, auto gc, int ncp, const double4 &x, int g,
+                        double4 &acc) __attribute__((always_inline)) {
+        constexpr int G = decltype(gc)::value;
+        constexpr bool MASK = decltype(mask)::value;
+        const double4 *sg = sb + g;
+        for (int j = 0; j < ncp; j += kP2PUnroll * G) {
+          // The kP2PUnroll pairs side by side (loads, then each step of the
+          // pair arithmetic for all of them), summed in source order.
+          double4 xj[kP2PUnroll];
+          double dx[kP2PUnroll], dy[kP2PUnroll], dz[kP2PUnroll], r2[kP2PUnroll], rs[kP2PUnroll];
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) xj[u] = sg[j + u * G];
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) {
+            dx[u] = x.x - xj[u].x;
+            dy[u] = x.y - xj[u].y;
+            dz[u] = x.z - xj[u].z;
+            r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          }
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) rs[u] = rsqrt_pos(r2[u]);
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) {
+            const double invr = MASK ? (r2[u] > 0 ? rs[u] : 0.0) : rs[u];
+            const double qinvr = xj[u].w * invr;
+            const double qinvr3 = qinvr * invr * invr;
+            acc.x += qinvr;
+            acc.y += dx[u] * qinvr3;
+            acc.z += dy[u] * qinvr3;
+            acc.w += dz[u] * qinvr3;
+          }
+        }
+      };
+      auto pass = [&](auto mask, int G, int ncp, const double4 &x, int g,
+                      double4 &acc) __attribute__((always_inline)) {
+        if (G == 1)
+          pass_g(mask, std::integral_constant<int, 1>{}, ncp, x, g, acc);
+        else if (G == 2)
+          pass_g(mask, std::integral_constant<int, 2>{}, ncp, x, g, acc);
+        else if (G == 4)
+          pass_g(mask, std::integral_constant<int, 4>{}, ncp, x, g, acc);
+        else
+          pass_g(mask, std::integral_constant<int, 8>{}, ncp, x, g, acc);
+      };
+      // Pad a filled tile of cnt sources to a multiple of kP2PUnroll G (G
+      // of either pass, at most 8; the stage size is a multiple of 64) with
+      // zero charges far away, which add exactly zero, then run both passes.
+      auto run = [&](auto mask, int cnt) __attribute__((always_inline)) {
         const int g4 = kP2PUnroll * (tp.r > 0 ? GB : GA);
         const int cntp = (cnt + g4 - 1) / g4 * g4;
         for (int k = cnt + lane; k < cntp; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
         __syncthreads();
-        // One pass over the stage with G (1, 2, 4 or 8) a compile-time
-        // constant and a wave-uniform trip count: the source addresses of a
-        // group are one base and immediate offsets.
         const int ncp = __builtin_amdgcn_readfirstlane(cntp);
-        auto pass_g = [&](auto mask, auto gc, const double4 &x, int g,
-                          double4 &acc) __attribute__((always_inline)) {
-          constexpr int G = decltype(gc)::value;
-          const double4 *sg = sb + g;
-          for (int j = 0; j < ncp; j += kP2PUnroll * G) {
-            // The kP2PUnroll pairs side by side (loads, then each step of
-            // the pair arithmetic for all of them), summed in source order.
-            double4 xj[kP2PUnroll];
-            double dx[kP2PUnroll], dy[kP2PUnroll], dz[kP2PUnroll], r2[kP2PUnroll], rs[kP2PUnroll];
-#pragma unroll
-            for (int u = 0; u < kP2PUnroll; u++) xj[u] = sg[j + u * G];
-#pragma unroll
-            for (int u = 0; u < kP2PUnroll; u++) {
-              dx[u] = x.x - xj[u].x;
-              dy[u] = x.y - xj[u].y;
-              dz[u] = x.z - xj[u].z;
-              r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-            }
-#pragma unroll
-            for (int u = 0; u < kP2PUnroll; u++) rs[u] = rsqrt_pos(r2[u]);
-#pragma unroll
-            for (int u = 0; u < kP2PUnroll; u++) {
-              const double invr = decltype(mask)::value ? (r2[u] > 0 ? rs[u] : 0.0) : rs[u];
-              const double qinvr = xj[u].w * invr;
-              const double qinvr3 = qinvr * invr * invr;
-              acc.x += qinvr;
-              acc.y += dx[u] * qinvr3;
-              acc.z += dy[u] * qinvr3;
-              acc.w += dz[u] * qinvr3;
-            }
-          }
-        };
-        auto pass = [&](auto mask, int G, const double4 &x, int g,
-                        double4 &acc) __attribute__((always_inline)) {
-          if (G == 1)
-            pass_g(mask, std::integral_constant<int, 1>{}, x, g, acc);
-          else if (G == 2)
-            pass_g(mask, std::integral_constant<int, 2>{}, x, g, acc);
-          else if (G == 4)
-            pass_g(mask, std::integral_constant<int, 4>{}, x, g, acc);
-          else
-            pass_g(mask, std::integral_constant<int, 8>{}, x, g, acc);
-        };
-        auto run = [&](auto mask) __attribute__((always_inline)) {
-          pass(mask, GA, xa, ga, acca);
-          if (tp.r > 0) pass(mask, GB, xb, gb, accb);
-        };
-        if (__any(own))
-          run(std::true_type{});
+        pass(mask, GA, ncp, xa, ga, acca);
+        if (tp.r > 0) pass(mask, GB, ncp, xb, gb, accb);
+      };
+      // The target leaf's own bodies first (masked), then the rest of the
+      // list with the own entry left out (the traversal lists every leaf
+      // against itself once).
+      for (int s0 = 0; s0 < tn; s0 += kP2PStage) {
+        const int cnt = min(kP2PStage, tn - s0);
+        __syncthreads();
+        for (int k = lane; k < cnt; k += 64) sb[k] = body[tb0 + s0 + k];
+        run(std::true_type{}, cnt);
+      }
+      SrcWalker w{e0, 0, e0, 0};
+      while (w.e < e1) {
+        // Fill: consecutive source leaves (or parts of one) up to kP2PStage
+        // bodies, each shifted by its periodic image.
+        bool imaged = false;
+        const int cnt = fill_stage<kP2PStage>(
+            w, e1, lane, src, img, cBody, cNBody, sd,
+            [&](int k, int b, int im) {
+              double4 v = body[b];
+              if (im != 13) {
+                double px, py, pz;
+                image_shift(im, cycle, px, py, pz);
+                v.x += px;
+                v.y += py;
+                v.z += pz;
+                imaged = true;
+              }
+              sb[k] = v;
+            },
+            c);
+        if (cnt == 0) continue;
+        if (__any(imaged))
+          run(std::true_type{}, cnt);
         else
-          run(std::false_type{});
+          run(std::false_type{}, cnt);
       }
       combine(acca, tp.TA, min(nt, tp.TA), tbase);
       if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
@@ -429,7 +443,7 @@ __device__ void rb_block(int lane, int c, int tb0, int tbase, int nt, int e0, in
 // KMAX: the most targets per lane; a leaf of more than 8 KMAX bodies runs as
 // several blocks (each walks its list once).
 template <int KMAX>
-__global__ void __launch_bounds__(64, 4)
+__global__ void __launch_bounds__(64)
     k_p2p_f64_rb(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
                  int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
                  const unsigned char *__restrict__ img, const int *__restrict__ cBody,
@@ -624,22 +638,28 @@ __global__ void __launch_bounds__(64)
       const Tgt ta = load_tgt(ia < nt, tb0 + tbase + ia);
       const Tgt tb = load_tgt(ib < tp.r, tb0 + tbase + tp.TA + ib);
       Acc acca, accb;
-      SrcWalker w{e0, 0, e0, 0};
-      while (w.e < e1) {
-        bool own = false;
-        const int cnt = fill_stage<kP2PStageF>(w, e1, lane, src, img, cBody, cNBody, sd, [&](int k, int b, int im) {
-          double px, py, pz;
-          image_shift(im, cycle, px, py, pz);
-          const double4 v = body[b];
-          const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
-          const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-          sb[k] = make_float4(hx, hy, hz, (float)v.w);
-          sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
-          own |= (b >= tb0 && b < tb0 + tn) || im != 13;
-        });
-        // Pad the tile to a multiple of kSeg G (G of either pass, at most 8;
-        // the stage size is a multiple of 64) with zero charges far away,
-        // which add exactly 0.
+      // Passes with G (1, 2, 4 or 8) a compile-time constant and a
+      // wave-uniform trip count, as in k_p2p_f64.
+      auto pass_g = [&](auto mask, auto gc, int ncp, const Tgt &t, int g,
+                        Acc &a) __attribute__((always_inline)) {
+        constexpr int G = decltype(gc)::value;
+        for (int j = 0; j < ncp; j += kSeg * G) segment(mask, gc, t, j + g, a);
+      };
+      auto pass = [&](auto mask, int G, int ncp, const Tgt &t, int g,
+                      Acc &a) __attribute__((always_inline)) {
+        if (G == 1)
+          pass_g(mask, std::integral_constant<int, 1>{}, ncp, t, g, a);
+        else if (G == 2)
+          pass_g(mask, std::integral_constant<int, 2>{}, ncp, t, g, a);
+        else if (G == 4)
+          pass_g(mask, std::integral_constant<int, 4>{}, ncp, t, g, a);
+        else
+          pass_g(mask, std::integral_constant<int, 8>{}, ncp, t, g, a);
+      };
+      // Pad a filled tile to a multiple of kSeg G (G of either pass, at
+      // most 8; the stage size is a multiple of 64) with zero charges far
+      // away, which add exactly 0, then run both passes.
+      auto run = [&](auto mask, int cnt) __attribute__((always_inline)) {
         const int g8 = kSeg * (tp.r > 0 ? GB : GA);
         const int cntp = (cnt + g8 - 1) / g8 * g8;
         for (int k = cnt + lane; k < cntp; k += 64) {
@@ -647,33 +667,45 @@ __global__ void __launch_bounds__(64)
           sbl[k] = make_float4(0, 0, 0, 0);
         }
         __syncthreads();
-        // Passes with G (1, 2, 4 or 8) a compile-time constant and a
-        // wave-uniform trip count, as in k_p2p_f64.
         const int ncp = __builtin_amdgcn_readfirstlane(cntp);
-        auto pass_g = [&](auto mask, auto gc, const Tgt &t, int g,
-                          Acc &a) __attribute__((always_inline)) {
-          constexpr int G = decltype(gc)::value;
-          for (int j = 0; j < ncp; j += kSeg * G) segment(mask, gc, t, j + g, a);
-        };
-        auto pass = [&](auto mask, int G, const Tgt &t, int g,
-                        Acc &a) __attribute__((always_inline)) {
-          if (G == 1)
-            pass_g(mask, std::integral_constant<int, 1>{}, t, g, a);
-          else if (G == 2)
-            pass_g(mask, std::integral_constant<int, 2>{}, t, g, a);
-          else if (G == 4)
-            pass_g(mask, std::integral_constant<int, 4>{}, t, g, a);
-          else
-            pass_g(mask, std::integral_constant<int, 8>{}, t, g, a);
-        };
-        auto run = [&](auto mask) __attribute__((always_inline)) {
-          pass(mask, GA, ta, ga, acca);
-          if (tp.r > 0) pass(mask, GB, tb, gb, accb);
-        };
-        if (__any(own))
-          run(std::true_type{});
+        pass(mask, GA, ncp, ta, ga, acca);
+        if (tp.r > 0) pass(mask, GB, ncp, tb, gb, accb);
+      };
+      // Source body v shifted by (px, py, pz), relative to the leaf centre,
+      // into slot k as FP32 hi (with the charge) and lo parts.
+      auto put = [&](int k, const double4 &v, double px, double py, double pz) __attribute__((always_inline)) {
+        const double rx = (v.x + px) - Xc.x, ry = (v.y + py) - Xc.y, rz = (v.z + pz) - Xc.z;
+        const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+        sb[k] = make_float4(hx, hy, hz, (float)v.w);
+        sbl[k] = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.0f);
+      };
+      // The target leaf's own bodies first (masked), then the rest of the
+      // list with the own entry left out (as in k_p2p_f64).
+      for (int s0 = 0; s0 < tn; s0 += kP2PStageF) {
+        const int cnt = min(kP2PStageF, tn - s0);
+        __syncthreads();
+        for (int k = lane; k < cnt; k += 64) put(k, body[tb0 + s0 + k], 0, 0, 0);
+        run(std::true_type{}, cnt);
+      }
+      SrcWalker w{e0, 0, e0, 0};
+      while (w.e < e1) {
+        bool imaged = false;
+        const int cnt = fill_stage<kP2PStageF>(
+            w, e1, lane, src, img, cBody, cNBody, sd,
+            [&](int k, int b, int im) {
+              double px = 0, py = 0, pz = 0;
+              if (im != 13) {
+                image_shift(im, cycle, px, py, pz);
+                imaged = true;
+              }
+              put(k, body[b], px, py, pz);
+            },
+            c);
+        if (cnt == 0) continue;
+        if (__any(imaged))
+          run(std::true_type{}, cnt);
         else
-          run(std::false_type{});
+          run(std::false_type{}, cnt);
       }
       combine(acca, tp.TA, min(nt, tp.TA), tbase);
       if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
@@ -846,7 +878,7 @@ __device__ void rb_block32(int lane, int c, int tb0, int tbase, int nt, int e0, 
 }
 
 template <int KMAX>
-__global__ void __launch_bounds__(64, 4)
+__global__ void __launch_bounds__(64)
     k_p2p_mixed_rb(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
                    int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
                    const unsigned char *__restrict__ img, const int *__restrict__ cBody,
