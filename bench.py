@@ -200,6 +200,20 @@ def bench_batched(args):
         dev.append(f.stats()["t_total"])
     ms = float(np.median(dev))
     ms_wall = float(np.median(wall))
+    # New charges on the same systems: the forest's numeric passes replayed
+    # from a HIP graph (exafmm_hip_evaluate_charges), results left on the
+    # device (wall time includes the charge upload and the device work).
+    qs = np.concatenate([sy[:, 3] for sy in systems])
+    f.evaluate_charges(qs * 0.5)
+    rdev, rwall = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        f.evaluate_charges(qs * (1.0 + 0.01 * k))
+        rwall.append((time.perf_counter() - t0) * 1e3)
+        rdev.append(f.stats()["t_total"])
+    f.evaluate_charges(qs)
+    out = f.results_batch()
+    rms = float(np.median(rdev))
     # Per-system error against FP64 direct summation on 64 sampled targets.
     errs = []
     for x, r in zip(systems, out):
@@ -211,14 +225,21 @@ def bench_batched(args):
     # One system of the same total N.
     g = exafmm_hip.FMM(P=8, theta=0.5, ncrit=ncrit)
     g.set_autotune(True)  # the single system gets its own tuning
-    g.set_bodies(unit_cube(total, 99))
+    xs = unit_cube(total, 99)
+    g.set_bodies(xs)
     g.evaluate()
     sdev = []
     for _ in range(steps):
         g.evaluate()
         sdev.append(g.stats()["t_total"])
+    g.evaluate_charges(xs[:, 3] * 0.5)
+    srdev = []
+    for k in range(steps):
+        g.evaluate_charges(xs[:, 3] * (1.0 + 0.01 * k))
+        srdev.append(g.stats()["t_total"])
     g.close()
     sms = float(np.median(sdev))
+    srms = float(np.median(srdev))
     line = {
         "metric": "batched FMM particles/s: 256 independent systems, N log-uniform in [4096, 262144], P=8, theta=0.5",
         "value": total / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": steps,
@@ -231,6 +252,11 @@ def bench_batched(args):
         "ms_wall_incl_host_copies": round(ms_wall, 3),
         "single_system_same_N": {"value": total / (sms * 1e-3), "ms": sms, "workload": "uniform cube, same N, ncrit"},
         "ratio_vs_single_system": (total / (ms * 1e-3)) / (total / (sms * 1e-3)),
+        "charges_only_replay": {
+            "workload": "same 256 systems, new charges: the forest's numeric passes replayed as one HIP graph launch",
+            "ms_device": round(rms, 4), "ms_wall_incl_host_copy": round(float(np.median(rwall)), 4),
+            "value": total / (rms * 1e-3), "single_system_same_N_ms": round(srms, 4),
+            "ratio_vs_single_system": srms / rms},
         "rel_l2_err_per_system": {"pot_max": float(errs[:, 0].max()), "acc_max": float(errs[:, 1].max()),
                                   "acc_median": float(np.median(errs[:, 1])), "targets_per_system": 64},
     }
@@ -335,6 +361,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--with-mixed", action="store_true", default=True)
     ap.add_argument("--no-mixed", dest="with_mixed", action="store_false")
+    ap.add_argument("--no-charges-replay", dest="charges_replay", action="store_false",
+                    help="skip the timing of new-charge re-evaluations replayed from a HIP graph")
     ap.add_argument("--no-isolated-rates", dest="isolated_rates", action="store_false",
                     help="skip the 3 extra evaluations that time the dominant kernels on an otherwise idle GPU")
     ap.add_argument("--copy-probe", action="store_true",
@@ -377,6 +405,24 @@ def main():
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         stats[-1]["num_p2p_bodies"] = f.count_p2p_bodies()  # outside the timed steps
         tuning = f.tuning()
+        # The same bodies with new charges (exafmm_hip_evaluate_charges): the
+        # numeric passes only, replayed from the recorded HIP graph.
+        charges = None
+        if args.charges_replay:
+            q0 = x[:, 3]
+            f.evaluate_charges(q0 * 0.5)  # first call: ordinary launches, then recorded
+            dev, wall = [], []
+            for k in range(args.steps):
+                t0 = time.perf_counter()
+                f.evaluate_charges(q0 * (1.0 + 0.01 * k))
+                wall.append((time.perf_counter() - t0) * 1e3)
+                dev.append(f.stats()["t_total"])
+            f.evaluate_charges(q0)  # leaves the results of the original charges
+            cms = float(np.median(dev))
+            charges = {"ms_device": round(cms, 4), "ms_wall_incl_host_copy": round(float(np.median(wall)), 4),
+                       "value": n / (cms * 1e-3), "unit": "particles/s",
+                       "workload": ("same bodies, new charges: P2M, M2M, M2L, L2L, L2P and P2P replayed from a "
+                                    "HIP graph (tree, lists and M2L set-up kept)")}
         # Accuracy against FP64 direct summation on 128 random targets.
         res = f.results()[tgt]
         ref = f.direct(tgt)
@@ -384,18 +430,19 @@ def main():
         err_acc = float(np.sqrt(np.sum((res[:, 1:] - ref[:, 1:]) ** 2) / np.sum(ref[:, 1:] ** 2)))
         f.close()
         med = {k: float(np.median([d[k] for d in stats])) for k in stats[0] if k.startswith("t_")}
-        return ms, stats[-1], med, err_pot, err_acc, tuning
+        return ms, stats[-1], med, err_pot, err_acc, tuning, charges
 
-    ms, s, med, err_pot, err_acc, fp64_tuning = run("fp64")
+    ms, s, med, err_pot, err_acc, fp64_tuning, fp64_charges = run("fp64")
     value = n / (ms * 1e-3)
     mixed = None
     mixed_med, mixed_tuning = None, None
     if args.precision == "mixed" or args.with_mixed:
-        mms, ms_stats, mmed, mpot, macc, mixed_tuning = run("mixed")
+        mms, ms_stats, mmed, mpot, macc, mixed_tuning, mixed_charges = run("mixed")
         mixed_med = mmed
         mixed = {"value": n / (mms * 1e-3), "unit": "particles/s", "ms_per_step": mms,
                  "speedup_vs_fp64": ms / mms, "rel_l2_err": {"pot": mpot, "acc": macc, "targets": 128},
                  "phases_ms": {k: round(v, 4) for k, v in mmed.items()},
+                 "charges_only_replay": mixed_charges,
                  "dtype": "f32 P2P (compensated) and f32 M2L tables; f64 centres, offsets, expansions"}
 
     # Per-kernel rates (algorithmic work / HIP-event time of that kernel).
@@ -492,6 +539,7 @@ def main():
                 "tiles": s["num_m2l_tiles"]},
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
         "phases_ms": {k: round(v, 4) for k, v in med.items()},
+        "charges_only_replay": fp64_charges,
         "counts": {"cells": s["num_cells"], "p2p_cell_pairs": s["num_p2p_pairs"],
                    "m2l_cell_pairs": s["num_m2l_pairs"], "p2p_body_pairs": p2p_pairs},
         "kernels_tflops": kernels_tflops,

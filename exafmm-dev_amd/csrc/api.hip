@@ -193,6 +193,7 @@ const char *exafmm_hip_last_error(const exafmm_hip_handle *H) { return H ? H->h.
 
 int exafmm_hip_set_config(exafmm_hip_handle *H, const exafmm_hip_config *cfg) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(cfg != nullptr, "set_config: null config");
     ensure(cfg->P == h.cfg.P && cfg->device == h.cfg.device && cfg->basis == h.cfg.basis,
            "set_config: P, basis and device are fixed at creation");
@@ -217,6 +218,7 @@ int exafmm_hip_set_config(exafmm_hip_handle *H, const exafmm_hip_config *cfg) {
 
 int exafmm_hip_set_bodies(exafmm_hip_handle *H, int n, const double *xyzq) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(n >= 0 && (n == 0 || xyzq), "set_bodies: bad arguments");
     h.n = n;
     h.nsys = 1;
@@ -229,6 +231,7 @@ int exafmm_hip_set_bodies(exafmm_hip_handle *H, int n, const double *xyzq) {
 
 int exafmm_hip_set_bodies_device(exafmm_hip_handle *H, int n, const double *d_xyzq) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(n >= 0 && (n == 0 || d_xyzq), "set_bodies_device: bad arguments");
     h.n = n;
     h.nsys = 1;
@@ -240,6 +243,7 @@ int exafmm_hip_set_bodies_device(exafmm_hip_handle *H, int n, const double *d_xy
 
 int exafmm_hip_get_bounds(exafmm_hip_handle *H, double *xmin3, double *xmax3) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     run_bounds(h);
     for (int d = 0; d < 3; d++) {
       if (xmin3) xmin3[d] = h.xmin[d];
@@ -250,6 +254,7 @@ int exafmm_hip_get_bounds(exafmm_hip_handle *H, double *xmin3, double *xmax3) {
 
 int exafmm_hip_build_tree(exafmm_hip_handle *H, const double *xmin3, const double *xmax3, int *ncells) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     if (xmin3 && xmax3) {
       for (int d = 0; d < 3; d++) {
         h.xmin[d] = xmin3[d];
@@ -262,12 +267,23 @@ int exafmm_hip_build_tree(exafmm_hip_handle *H, const double *xmin3, const doubl
   });
 }
 
-int exafmm_hip_upward_pass(exafmm_hip_handle *H) { return guarded(H, [&](Handle &h) { do_upward(h); }); }
+int exafmm_hip_upward_pass(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) {
+    drop_graph(h);
+    do_upward(h);
+  });
+}
 
-int exafmm_hip_traverse(exafmm_hip_handle *H) { return guarded(H, [&](Handle &h) { do_traverse(h); }); }
+int exafmm_hip_traverse(exafmm_hip_handle *H) {
+  return guarded(H, [&](Handle &h) {
+    drop_graph(h);
+    do_traverse(h);
+  });
+}
 
 int exafmm_hip_downward_pass(exafmm_hip_handle *H) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(h.have_trav, "downward_pass: traversal not run");
     run_downward(h);
   });
@@ -395,7 +411,10 @@ static void evaluate_tuned(Handle &h) {
 }
 
 int exafmm_hip_evaluate(exafmm_hip_handle *H) {
-  return guarded(H, [&](Handle &h) { evaluate_tuned(h); });
+  return guarded(H, [&](Handle &h) {
+    drop_graph(h);
+    evaluate_tuned(h);
+  });
 }
 
 int exafmm_hip_set_autotune(exafmm_hip_handle *H, int enable) {
@@ -415,6 +434,7 @@ int exafmm_hip_get_tuning(exafmm_hip_handle *H, int *ncrit, int *p2p_waves_per_c
 int exafmm_hip_set_tuning(exafmm_hip_handle *H, int ncrit, int p2p_waves_per_cu, int p2p_targets_per_lane,
                           int m2l_groups_per_cu) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(ncrit > 0 && p2p_waves_per_cu >= 0 && m2l_groups_per_cu >= 0 &&
                (p2p_targets_per_lane == 1 || (p2p_targets_per_lane >= 3 && p2p_targets_per_lane <= 5)),
            "set_tuning: bad parameters");
@@ -424,6 +444,7 @@ int exafmm_hip_set_tuning(exafmm_hip_handle *H, int ncrit, int p2p_waves_per_cu,
 
 int exafmm_hip_evaluate_batch(exafmm_hip_handle *H, int nsys, const int *n, const double *xyzq, double *trg) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(nsys >= 1 && n && xyzq && trg, "evaluate_batch: bad arguments");
     ensure(h.cfg.builder == 0 && h.cfg.images == 0 && h.cfg.basis == 0,
            "evaluate_batch: uniform builder, no periodic images, spherical basis");
@@ -524,6 +545,7 @@ int exafmm_hip_get_list_counts(exafmm_hip_handle *H, int64_t *np2p, int64_t *nm2
 
 int exafmm_hip_get_lists(exafmm_hip_handle *H, int *p2p_i, int *p2p_j, int *m2l_i, int *m2l_j) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(h.have_trav, "get_lists: traversal not run");
     const int nc = h.ncells;
     auto expand = [&](DBuf<int> &off, DBuf<int> &src, int64_t np, int *ti, int *tj) {
@@ -576,6 +598,7 @@ int exafmm_hip_dipole_correction(exafmm_hip_handle *H, const double *dipole, int
 int exafmm_hip_ewald(exafmm_hip_handle *H, int ksize, double alpha, double sigma, double cutoff, double cycle,
                      int parts, double *trg) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(h.have_tree, "ewald: tree not built");
     ensure(trg && ksize >= 0 && alpha > 0 && sigma > 0 && cutoff > 0 && cycle > 0 && parts >= 0 && parts <= 7,
            "ewald: bad arguments");

@@ -217,6 +217,7 @@ extern "C" {
 int exafmm_hip_cell_p2p(exafmm_hip_handle *H, int ni, const double *ti, double *trg_i, int nj, const double *sj,
                         double *trg_j, double eps2, const double *Xperiodic, int mutual) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(ni >= 0 && nj >= 0, "cell_p2p: negative body count");
     ensure((ni == 0 || (ti && trg_i)) && (nj == 0 || sj) && (!mutual || nj == 0 || trg_j), "cell_p2p: null array");
     if (ni == 0 || nj == 0) return;
@@ -246,6 +247,7 @@ int exafmm_hip_cell_p2p(exafmm_hip_handle *H, int ni, const double *ti, double *
 
 int exafmm_hip_cell_p2p_self(exafmm_hip_handle *H, int n, const double *xyzq, double *trg, double eps2) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(n >= 0 && (n == 0 || (xyzq && trg)), "cell_p2p_self: bad arguments");
     if (n == 0) return;
     h.co_body.resize(n);
@@ -261,6 +263,7 @@ int exafmm_hip_cell_p2p_self(exafmm_hip_handle *H, int n, const double *xyzq, do
 
 int exafmm_hip_cell_p2m(exafmm_hip_handle *H, int n, const double *xyzq, const double *X, double *M) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(n >= 0 && X && M && (n == 0 || xyzq), "cell_p2m: bad arguments");
     if (n == 0) return;
     // One leaf holding the n bodies: P2M into a zeroed M, then added.
@@ -288,6 +291,7 @@ int exafmm_hip_cell_p2m(exafmm_hip_handle *H, int n, const double *xyzq, const d
 int exafmm_hip_cell_m2m(exafmm_hip_handle *H, int nchild, const double *Xc, const double *Mc, const double *X,
                         double *M) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(nchild >= 0 && X && M && (nchild == 0 || (Xc && Mc)), "cell_m2m: bad arguments");
     if (nchild == 0) return;
     // Cell 0 is the parent, cells 1 .. nchild its children.
@@ -321,6 +325,7 @@ int exafmm_hip_cell_m2m(exafmm_hip_handle *H, int nchild, const double *Xc, cons
 int exafmm_hip_cell_m2l(exafmm_hip_handle *H, const double *Xi, const double *Mi, double *Li, const double *Xj,
                         const double *Mj, double *Lj, const double *Xperiodic, int mutual) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(Xi && Li && Xj && Mj, "cell_m2l: null argument");
     ensure(!(mutual || h.cfg.mass) || Mi, "cell_m2l: Mi is needed with mutual and in MASS mode");
     ensure(!mutual || Lj, "cell_m2l: Lj is needed with mutual");
@@ -364,6 +369,7 @@ int exafmm_hip_cell_m2l(exafmm_hip_handle *H, const double *Xi, const double *Mi
 int exafmm_hip_cell_l2l(exafmm_hip_handle *H, const double *Xp, const double *Lp, const double *X, const double *M,
                         double *L) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(Xp && Lp && X && L, "cell_l2l: null argument");
     ensure(!h.cfg.mass || M, "cell_l2l: M (the cell's M0) is needed in MASS mode");
     // Cell 0 = the parent, cell 1 = the cell, marked as having a child so
@@ -400,6 +406,7 @@ int exafmm_hip_cell_l2l(exafmm_hip_handle *H, const double *Xp, const double *Lp
 int exafmm_hip_cell_l2p(exafmm_hip_handle *H, int n, const double *xyzq, const double *X, const double *L,
                         double *trg) {
   return guarded(H, [&](Handle &h) {
+    drop_graph(h);
     ensure(n >= 0 && X && L && (n == 0 || (xyzq && trg)), "cell_l2p: bad arguments");
     if (n == 0) return;
     // One leaf holding the n bodies; TRG /= SRC on the device, then the

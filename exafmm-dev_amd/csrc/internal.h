@@ -68,6 +68,15 @@ struct cplx {
   double re, im;
 };
 
+// A CSR interaction list by target cell (targets with work listed in tgt).
+struct M2LList {
+  const int *tgt;
+  int ntgt;
+  const int *off;
+  const int *src;
+  const unsigned char *img;
+};
+
 struct Handle {
   exafmm_hip_config cfg;
   int nterm = 0;
@@ -172,6 +181,8 @@ struct Handle {
   bool lat_tile_alt = false;     // sorted tile keys in the second half of lat_tkey / lat_tval
   DBuf<double> lat_T, lat_Ms;
   DBuf<float> lat_Tf, lat_Msf;
+  int lat_run_ntiles = 0, lat_run_ngrid = 0;  // the last contraction's tiles and grid (0 tiles: none ran)
+  M2LList m2l_last{};                          // the pairs the last run_m2l gave the per-pair kernel
   int64_t lat_stats_pairs = 0;
   int lat_stats_noff = 0, lat_stats_tiles = 0;
   bool lat_timed = false, pair_timed = false;  // ev[12..13]: MFMA kernel, ev[14..15]: per-pair kernel
@@ -210,6 +221,16 @@ struct Handle {
   // Caller-order results staged on the device (get_results, evaluate_batch).
   DBuf<double4> out4;
 
+  // Re-evaluation of the last geometry with new charges (evaluate_charges):
+  // the charges in caller order, and the numeric passes recorded once as a
+  // HIP graph (graph_ok) and replayed; every entry point that may change the
+  // tree, the lists or a buffer the graph uses drops it (drop_graph).
+  DBuf<double> qin;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
+  hipEvent_t ev_fork = nullptr, ev_gmult = nullptr, ev_gp2p = nullptr;
+
   // Autotuning (exafmm_hip_set_autotune): for each input signature, the
   // fastest (ncrit, P2P waves per CU, P2P targets per lane, contraction
   // workgroups per CU) among timed candidate evaluations.
@@ -224,15 +245,6 @@ struct Handle {
 };
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
-
-// A CSR interaction list by target cell (targets with work listed in tgt).
-struct M2LList {
-  const int *tgt;
-  int ntgt;
-  const int *off;
-  const int *src;
-  const unsigned char *img;
-};
 
 // Device arrays of a small cell set that the per-cell operators of the
 // C-ABI build and hand to the pass kernels: the cells to process (`cells`,
@@ -289,6 +301,14 @@ void group_pairs(Handle &h, int64_t np, DBuf<int> &tgt, DBuf<int> &src, DBuf<uns
                  DBuf<int> &src_out, DBuf<unsigned char> &img_out, DBuf<int> &tlist, int &ntl);
 void run_p2p(Handle &h, hipStream_t st);
 void run_m2l(Handle &h);
+// The lattice contraction and reduction / the whole M2L of the last
+// run_m2l again, on the current multipoles (its set-up reused).
+void lattice_replay(Handle &h);
+void run_m2l_replay(Handle &h);
+// P2M and M2M of every level on stream st (the radii are left as they are).
+void run_multipoles(Handle &h, hipStream_t st);
+// Destroy the recorded charges-only graph (see Handle::graph).
+void drop_graph(Handle &h);
 void run_downward(Handle &h);
 void run_direct(Handle &h, int nt, const int *target_ibody, double *trg);
 void run_direct_arrays(Handle &h, int nt, const double *t_xyzq, int ns, const double *s_xyzq, double *trg);
@@ -320,6 +340,11 @@ void launch_downward_cart_cells(Handle &h, const CellSetView &v);
 // (spherical per-pair kernel, or the Cartesian kernel for basis 1).
 void launch_m2l_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L);
 void launch_m2l_cart_arrays(Handle &h, const M2LList &lst, const double4 *cX, const cplx *M, cplx *L);
+
+// Zero / copy device memory with a kernel (bytes a multiple of 8), so the
+// work can be recorded in a HIP graph (no memset or copy nodes).
+void zero_device(void *p, size_t bytes, hipStream_t st);
+void copy_device(void *d, const void *s, size_t bytes, hipStream_t st);
 
 // Scan / sort helpers (hipCUB) in util.hip.
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n);

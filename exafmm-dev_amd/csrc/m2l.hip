@@ -361,12 +361,30 @@ void run_m2l(Handle &h) {
     lst = M2LList{h.m2l_tgt.p, h.nm2l_tgt, h.m2l_off.p, h.m2l_src.p, h.m2l_img.p};
   }
   join_multipoles(h);  // (the lattice path joined them before its contraction)
+  h.m2l_last = lst;
   switch (h.cfg.P) {
     case 4: m2l_impl<4>(h, lst); break;
     case 6: m2l_impl<6>(h, lst); break;
     case 8: m2l_impl<8>(h, lst); break;
     case 10: m2l_impl<10>(h, lst); break;
     case 12: m2l_impl<12>(h, lst); break;
+    default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
+  }
+}
+
+// The M2L of the last run_m2l again on the current multipoles: the lattice
+// contraction from its set-up, then the per-pair kernel on the pairs the last
+// run gave it (no host waits; spherical basis).
+void run_m2l_replay(Handle &h) {
+  if (h.nm2l == 0) return;
+  lattice_replay(h);
+  const bool f32 = h.cfg.precision == 1;
+  switch (h.cfg.P) {
+    case 4: m2l_launch<4>(h, h.m2l_last, h.cX.p, h.M.p, h.L.p, f32, false); break;
+    case 6: m2l_launch<6>(h, h.m2l_last, h.cX.p, h.M.p, h.L.p, f32, false); break;
+    case 8: m2l_launch<8>(h, h.m2l_last, h.cX.p, h.M.p, h.L.p, f32, false); break;
+    case 10: m2l_launch<10>(h, h.m2l_last, h.cX.p, h.M.p, h.L.p, f32, false); break;
+    case 12: m2l_launch<12>(h, h.m2l_last, h.cX.p, h.M.p, h.L.p, f32, false); break;
     default: throw Error(EXAFMM_HIP_EUNSUP, "P must be one of 4, 6, 8, 10, 12");
   }
 }

@@ -629,6 +629,27 @@ static void grow_keep_rows(Handle &h, DBuf<R> &b, size_t want, size_t keep) {
   nb.cap = 0;
 }
 
+// The contraction and the reduction of the last lattice set-up (its step
+// lists, slots and tiles), on the main stream.
+template <int P>
+static void lattice_contract(Handle &h) {
+  const int nitems = h.lat_run_ntiles * kLatSeg;
+  if (h.cfg.precision == 1)
+    hipLaunchKernelGGL((k_m2l_mfma<P, float>), dim3(h.lat_run_ngrid), dim3(256), 0, h.stream, h.ncells, nitems,
+                       h.lat_step0.p, h.lat_alist.p, h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
+  else
+    hipLaunchKernelGGL((k_m2l_mfma<P, double>), dim3(h.lat_run_ngrid), dim3(256), 0, h.stream, h.ncells, nitems,
+                       h.lat_step0.p, h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
+  EXA_HIP(hipGetLastError());
+}
+
+template <int P>
+static void lattice_reduce(Handle &h) {
+  hipLaunchKernelGGL(k_lat_reduce<P>, dim3(h.lat_run_ntiles, MfmaShape<P>::RB), dim3(256), 0, h.stream,
+                     h.lat_tile_tgt.p, h.cRb.p, h.lat_part.p, h.L.p);
+  EXA_HIP(hipGetLastError());
+}
+
 template <int P>
 static void lattice_begin(Handle &h) {
   const int nc = h.ncells;
@@ -816,6 +837,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
   // 6. The contraction (timed by ev[12..13]) and the reduction.
   const bool run_mfma = noff > 0 && ntiles > 0 && nsteps_all > 0;
   join_multipoles(h);  // the set-up above did not need them
+  h.lat_run_ntiles = 0;
   if (run_mfma) {
     h.lat_part.resize((size_t)ntiles * kLatSeg * S::RB * 16 * 64);
     // One workgroup per item, or lat_wg_cu workgroups per CU looping over
@@ -823,19 +845,13 @@ static void lattice_run(Handle &h, M2LList &irr) {
     const int nitems = ntiles * kLatSeg;
     int ngrid = nitems;
     if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, std::max(kLatSeg, (h.lat_wg_cu * h.ncu) / kLatSeg * kLatSeg));
-    const dim3 grid(ngrid);
+    h.lat_run_ntiles = ntiles;
+    h.lat_run_ngrid = ngrid;
     EXA_HIP(hipEventRecord(h.ev[12], h.stream));
     h.lat_timed = true;
-    if (f32)
-      hipLaunchKernelGGL((k_m2l_mfma<P, float>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
-                         h.lat_alist.p, h.lat_slot.p, h.lat_Tf.p, h.lat_Msf.p, h.lat_part.p);
-    else
-      hipLaunchKernelGGL((k_m2l_mfma<P, double>), grid, dim3(256), 0, h.stream, nc, nitems, h.lat_step0.p,
-                         h.lat_alist.p, h.lat_slot.p, h.lat_T.p, h.lat_Ms.p, h.lat_part.p);
+    lattice_contract<P>(h);
     EXA_HIP(hipEventRecord(h.ev[13], h.stream));
-    hipLaunchKernelGGL(k_lat_reduce<P>, dim3(ntiles, S::RB), dim3(256), 0, h.stream, h.lat_tile_tgt.p, h.cRb.p,
-                       h.lat_part.p, h.L.p);
-    EXA_HIP(hipGetLastError());
+    lattice_reduce<P>(h);
   }
   EXA_HIP(hipEventSynchronize(h.ev_cnt));
   const int nirr = *h.hpin;
@@ -894,8 +910,22 @@ void m2l_begin(Handle &h) {
   EXA_HIP(hipGetLastError());
 }
 
+// The lattice contraction and reduction again on the current multipoles,
+// with the set-up of the last run_m2l_lattice (same tree and lists).
+void lattice_replay(Handle &h) {
+  if (!lattice_applies(h) || h.nm2l == 0 || h.lat_run_ntiles == 0) return;
+  switch (h.cfg.P) {
+    case 4: lattice_contract<4>(h); lattice_reduce<4>(h); break;
+    case 6: lattice_contract<6>(h); lattice_reduce<6>(h); break;
+    case 8: lattice_contract<8>(h); lattice_reduce<8>(h); break;
+    case 10: lattice_contract<10>(h); lattice_reduce<10>(h); break;
+    case 12: lattice_contract<12>(h); lattice_reduce<12>(h); break;
+  }
+}
+
 bool run_m2l_lattice(Handle &h, M2LList &irr) {
   h.lat_timed = false;
+  h.lat_run_ntiles = 0;
   h.lat_stats_pairs = 0;
   h.lat_stats_noff = h.lat_stats_tiles = 0;
   if (!lattice_applies(h) || h.nm2l == 0) {

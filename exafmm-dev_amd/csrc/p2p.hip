@@ -42,38 +42,48 @@ constexpr int kP2PUnroll = 2;
 static_assert(kP2PStage % (8 * kP2PUnroll) == 0, "stages are padded to multiples of kP2PUnroll G, G <= 8");
 constexpr double kFar = 1e100;  // position of the zero-charge padding bodies
 
-// 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
-// higher-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, which
-// takes the estimate's ~2^-22 relative error below FP64 rounding.  (No
-// special-case fix-ups: the caller selects 0 for x == 0.)
+// 1/sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64, measured
+// on MI355X at most 2^-24.2 relative error over 2^22 arguments spanning 60
+// binades) refined by one Newton step y (1 + e/2), e = 1 - x y^2, whose
+// error is -1.5 times the square of the estimate's: below 4.4e-15 relative.
+// (No special-case fix-ups: the caller selects 0 for x == 0.)
 __device__ inline double rsqrt_pos(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double e = fma(-x * y, y, 1.0);
-  return fma(y * e, fma(e, 0.375, 0.5), y);
+  return fma(y * e, 0.5, y);
 }
 
 // Walker over a target's source list (CSR entries [e, e1)).  The
 // descriptors of up to 64 consecutive entries -- first body, body count,
 // image -- and the exclusive prefix of their counts are loaded in one
-// parallel step; a stage fill then maps each stage slot to (entry, body) by
-// binary search over that prefix, so all body loads of a fill are
-// independent of each other (no per-leaf dependent load chain).  The
-// workgroup is one wave; every call is uniform across it.
+// parallel step.  A stage fill maps its slots to entries through a slot map:
+// every entry that starts inside the stage marks its first slot, and a
+// running maximum over the slots (four per lane, then across the wave) gives
+// each slot its entry; the body loads of a lane's slots are then issued
+// together.  The workgroup is one wave; every call is uniform across it.
 struct SrcWalker {
   int e, sbase, dbase, dcount;
 };
+
+constexpr int kFillMax = 256;  // largest stage (slots per fill)
 
 struct SrcDesc {
   int b0[64];
   int pre[65];
   unsigned char img[64];
+  int map[kFillMax];  // slot -> entry (index into b0 / pre / img)
 };
 
 // skip: a cell whose unshifted entry (image 13) is left out (-1: none).
-template <int STAGE, typename Put>
+// put(k, v, im) stores body v (x, y, z, q) with image im into slot k; NB
+// body loads are in flight at a time (registers against latency).
+template <int STAGE, int NB = 4, typename Put>
 __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__restrict__ src,
                                  const unsigned char *__restrict__ img, const int *__restrict__ cBody,
-                                 const int *__restrict__ cNBody, SrcDesc &sd, Put put, int skip = -1) {
+                                 const int *__restrict__ cNBody, const double4 *__restrict__ body, SrcDesc &sd,
+                                 Put put, int skip = -1) {
+  static_assert(STAGE == kFillMax, "the slot map covers one full stage, four slots per lane");
+  static_assert((STAGE / 64) % NB == 0, "whole batches of loads");
   __syncthreads();  // the previous stage (and descriptor batch) are consumed
   if (w.e >= w.dbase + w.dcount) {
     w.dbase = w.e;
@@ -99,17 +109,53 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
   const int q0 = sd.pre[first] + w.sbase;
   const int qend = sd.pre[64];
   const int cnt = min(STAGE, qend - q0);
-  for (int k = lane; k < cnt; k += 64) {
-    const int q = q0 + k;
-    int lo = first, hi = w.dcount;  // largest i in [first, dcount) with pre[i] <= q
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sd.pre[mid] <= q)
-        lo = mid;
-      else
-        hi = mid;
+  // Slot map: entry `first` from slot 0, each later entry with bodies from
+  // its first slot in the stage; a running maximum fills the rest.
+  int4 *m4 = reinterpret_cast<int4 *>(sd.map);
+  m4[lane] = make_int4(lane == 0 ? first : -1, -1, -1, -1);
+  {
+    const int e = first + 1 + lane;
+    if (e < w.dcount) {
+      const int s0 = sd.pre[e] - q0;
+      if (s0 < cnt && sd.pre[e + 1] > sd.pre[e]) sd.map[s0] = e;
     }
-    put(k, sd.b0[lo] + (q - sd.pre[lo]), sd.img[lo]);
+  }
+  __syncthreads();
+  int4 mv = m4[lane];
+  mv.y = max(mv.y, mv.x);
+  mv.z = max(mv.z, mv.y);
+  mv.w = max(mv.w, mv.z);
+  int run = mv.w;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(run, d);
+    if (lane >= d) run = max(run, t);
+  }
+  int before = __shfl_up(run, 1);
+  if (lane == 0) before = -1;
+  mv.x = max(mv.x, before);
+  mv.y = max(mv.y, before);
+  mv.z = max(mv.z, before);
+  mv.w = max(mv.w, before);
+  m4[lane] = mv;
+  __syncthreads();
+  // The lane's slots k = lane + 64 u, NB at a time: their body loads first
+  // (slots past the stage's end load a body of the stage and store nothing),
+  // then the stores.
+#pragma unroll 1
+  for (int u0 = 0; u0 < STAGE / 64; u0 += NB) {
+    double4 v[NB];
+    int ims[NB];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const int k = lane + 64 * (u0 + u);
+      const int e = k < cnt ? sd.map[k] : first;
+      const int b = k < cnt ? sd.b0[e] + (q0 + k - sd.pre[e]) : sd.b0[e] + (q0 - sd.pre[e]);
+      ims[u] = sd.img[e];
+      v[u] = body[b];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++)
+      if (lane + 64 * (u0 + u) < cnt) put(lane + 64 * (u0 + u), v[u], ims[u]);
   }
   // Advance to the first unconsumed body.
   const int qn = q0 + cnt;
@@ -117,7 +163,7 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
     w.e = w.dbase + w.dcount;
     w.sbase = 0;
   } else {
-    int lo = first, hi = w.dcount;
+    int lo = first, hi = w.dcount;  // largest i in [first, dcount) with pre[i] <= qn
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (sd.pre[mid] <= qn)
@@ -282,9 +328,9 @@ __global__ void __launch_bounds__(64)
         // bodies, each shifted by its periodic image.
         bool imaged = false;
         const int cnt = fill_stage<kP2PStage>(
-            w, e1, lane, src, img, cBody, cNBody, sd,
-            [&](int k, int b, int im) {
-              double4 v = body[b];
+            w, e1, lane, src, img, cBody, cNBody, body, sd,
+            [&](int k, const double4 &vb, int im) {
+              double4 v = vb;
               if (im != 13) {
                 double px, py, pz;
                 image_shift(im, cycle, px, py, pz);
@@ -400,11 +446,11 @@ __device__ void rb_block(int lane, int c, int tb0, int tbase, int nt, int e0, in
   while (w.e < e1) {
     bool imaged = false;
     const int cnt = fill_stage<kRbStage>(
-        w, e1, lane, src, img, cBody, cNBody, sd,
-        [&](int k, int b, int im) {
+        w, e1, lane, src, img, cBody, cNBody, body, sd,
+        [&](int k, const double4 &vb, int im) {
           double px, py, pz;
           image_shift(im, cycle, px, py, pz);
-          double4 v = body[b];
+          double4 v = vb;
           v.x += px;
           v.y += py;
           v.z += pz;
@@ -690,15 +736,15 @@ __global__ void __launch_bounds__(64)
       SrcWalker w{e0, 0, e0, 0};
       while (w.e < e1) {
         bool imaged = false;
-        const int cnt = fill_stage<kP2PStageF>(
-            w, e1, lane, src, img, cBody, cNBody, sd,
-            [&](int k, int b, int im) {
+        const int cnt = fill_stage<kP2PStageF, 1>(
+            w, e1, lane, src, img, cBody, cNBody, body, sd,
+            [&](int k, const double4 &vb, int im) {
               double px = 0, py = 0, pz = 0;
               if (im != 13) {
                 image_shift(im, cycle, px, py, pz);
                 imaged = true;
               }
-              put(k, body[b], px, py, pz);
+              put(k, vb, px, py, pz);
             },
             c);
         if (cnt == 0) continue;
@@ -841,11 +887,11 @@ __device__ void rb_block32(int lane, int c, int tb0, int tbase, int nt, int e0, 
   while (w.e < e1) {
     bool imaged = false;
     const int cnt = fill_stage<kRbStage>(
-        w, e1, lane, src, img, cBody, cNBody, sd,
-        [&](int k, int b, int im) {
+        w, e1, lane, src, img, cBody, cNBody, body, sd,
+        [&](int k, const double4 &vb, int im) {
           double px, py, pz;
           image_shift(im, cycle, px, py, pz);
-          put(k, body[b], px, py, pz);
+          put(k, vb, px, py, pz);
           imaged |= im != 13;
         },
         c);
@@ -931,7 +977,7 @@ __global__ void k_count_body_pairs(const int *__restrict__ tgt_list, int ntl, co
 }
 
 void run_p2p(Handle &h, hipStream_t st) {
-  EXA_HIP(hipMemsetAsync(h.trg4.p, 0, (size_t)h.n * sizeof(double4), st));
+  zero_device(h.trg4.p, (size_t)h.n * sizeof(double4), st);
   if (h.np2p_tgt == 0) return;
   // A resident grid of p2p_wpc waves per CU that claims target leaves from a
   // counter (0: one wave per leaf).  A resident grid leaves the rest of each

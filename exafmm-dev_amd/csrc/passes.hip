@@ -449,6 +449,8 @@ void run_upward_mult(Handle &h) {
   EXA_HIP(hipEventRecord(h.ev_mult, h.stream3));
 }
 
+void run_multipoles(Handle &h, hipStream_t st) { EXA_DISPATCH_P3(upward_impl, h, 1, st); }
+
 void run_downward(Handle &h) {
   if (h.cfg.basis == 1) return run_downward_cart(h);
   EXA_DISPATCH_P(downward_impl, h);

@@ -184,7 +184,7 @@ static void periodic_impl(Handle &h) {
   constexpr int NT = Expn<P>::NT;
   h.per_M.resize(NT);
   h.per_part.resize(26 * NT);
-  EXA_HIP(hipMemcpyAsync(h.per_M.p, h.M.p, NT * sizeof(cplx), hipMemcpyDeviceToDevice, h.stream));
+  copy_device(h.per_M.p, h.M.p, NT * sizeof(cplx), h.stream);
   double cycle = h.cfg.cycle;
   for (int lev = 0; lev < h.cfg.images - 1; lev++) {
     hipLaunchKernelGGL((k_periodic_m2l<P>), dim3(26), dim3(64), 0, h.stream, h.cX.p, h.M.p, h.per_M.p, cycle,

@@ -1,6 +1,8 @@
 // Scan and sort helpers over hipCUB (rocPRIM) device primitives, plus the
 // handle destructor.  Sorting and scanning are bookkeeping around the hot
 // path (tree build and list grouping); the hot kernels are hand-written.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -9,6 +11,10 @@
 namespace exafmm {
 
 Handle::~Handle() {
+  drop_graph(*this);
+  for (hipEvent_t e : {ev_fork, ev_gmult, ev_gp2p})
+    if (e) (void)hipEventDestroy(e);
+  qin.release();
   in4.release();
   body4.release();
   bidx.release();
@@ -68,6 +74,33 @@ Handle::~Handle() {
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
   if (stream3) (void)hipStreamDestroy(stream3);
+}
+
+// Zero / copy device memory by kernel (graph-capturable; bytes a multiple of 8).
+__global__ void k_zero_u64(unsigned long long *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void k_copy_u64(unsigned long long *__restrict__ d, const unsigned long long *__restrict__ s, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+void zero_device(void *p, size_t bytes, hipStream_t st) {
+  const int64_t n = (int64_t)(bytes / 8);
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_zero_u64, dim3((int)std::min<int64_t>(div_up(n, 256), 8192)), dim3(256), 0, st,
+                     (unsigned long long *)p, n);
+  EXA_HIP(hipGetLastError());
+}
+
+void copy_device(void *d, const void *s, size_t bytes, hipStream_t st) {
+  const int64_t n = (int64_t)(bytes / 8);
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_copy_u64, dim3((int)std::min<int64_t>(div_up(n, 256), 8192)), dim3(256), 0, st,
+                     (unsigned long long *)d, (const unsigned long long *)s, n);
+  EXA_HIP(hipGetLastError());
 }
 
 void exclusive_scan_int(Handle &h, const int *in, int *out, int n) { exclusive_scan_int(in, out, n, h.stream, h.tmp); }

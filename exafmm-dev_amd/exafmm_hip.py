@@ -28,6 +28,7 @@ EXPORTS = (
     "exafmm_hip_cell_p2p", "exafmm_hip_cell_p2p_self", "exafmm_hip_cell_p2m", "exafmm_hip_cell_m2m",
     "exafmm_hip_cell_m2l", "exafmm_hip_cell_l2l", "exafmm_hip_cell_l2p",
     "exafmm_hip_set_autotune", "exafmm_hip_get_tuning", "exafmm_hip_set_tuning",
+    "exafmm_hip_evaluate_charges", "exafmm_hip_evaluate_charges_device",
 )
 
 
@@ -113,6 +114,8 @@ def load():
     L.exafmm_hip_get_stats.argtypes = [vp, P(Stats)]
     L.exafmm_hip_count_p2p_bodies.argtypes = [vp, P(ctypes.c_int64)]
     L.exafmm_hip_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    L.exafmm_hip_evaluate_charges.argtypes = [vp, ip, vp]
+    L.exafmm_hip_evaluate_charges_device.argtypes = [vp, ip, vp]
     L.exafmm_hip_get_dipole.argtypes = [vp, vp, vp]
     L.exafmm_hip_dipole_correction.argtypes = [vp, vp, ip, dp]
     L.exafmm_hip_ewald.argtypes = [vp, ip, dp, dp, dp, dp, ip, vp]
@@ -317,7 +320,23 @@ class FMM:
         out = np.empty_like(x)
         self._check(self.lib.exafmm_hip_evaluate_batch(self.h, len(arrs), n.ctypes.data, x.ctypes.data,
                                                        out.ctypes.data), "evaluate_batch")
+        self.n = x.shape[0]
+        self._batch = n
         return np.split(out, np.cumsum(n)[:-1])
+
+    def evaluate_charges(self, q):
+        """Re-evaluate the last evaluated bodies with new charges q (caller
+        order; for a batch, the systems' charges as a list or concatenated):
+        the numeric passes only, replayed from a HIP graph after the first
+        call.  Results as after evaluate (results(), or results_batch())."""
+        if isinstance(q, (list, tuple)):
+            q = np.concatenate([np.asarray(v, np.float64).reshape(-1) for v in q])
+        a = np.ascontiguousarray(q, np.float64).reshape(-1)
+        self._check(self.lib.exafmm_hip_evaluate_charges(self.h, a.shape[0], _ptr(a)), "evaluate_charges")
+
+    def results_batch(self):
+        """results() split per system of the last evaluate_batch."""
+        return np.split(self.results(), np.cumsum(self._batch)[:-1])
 
     def set_autotune(self, enable=True):
         """Tune ncrit, the P2P grid and targets per lane, and the contraction

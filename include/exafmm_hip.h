@@ -186,6 +186,26 @@ int exafmm_hip_ewald(exafmm_hip_handle *h, int ksize, double alpha, double sigma
  * systems at once. */
 int exafmm_hip_evaluate_batch(exafmm_hip_handle *h, int nsys, const int *n, const double *xyzq, double *trg);
 
+/* Re-evaluate the last evaluated geometry (evaluate, the pass-level calls or
+ * evaluate_batch) with new charges q: n = the body count of that evaluation,
+ * q in the caller's body order (for a batch, the systems concatenated as in
+ * evaluate_batch).  The tree, radii, lists and lattice M2L set-up are kept;
+ * the numeric passes (P2M, M2M, M2L, periodic far field, L2L, L2P, P2P) run
+ * with the new charges and leave the results as evaluate does (get_results,
+ * results_device).  The first call after an evaluation runs them as ordinary
+ * launches and records them as a HIP graph; later calls launch that graph
+ * (no host work between kernels).  Results are bit-identical to evaluate()
+ * on the same bodies with these charges.  Any call that changes the bodies,
+ * the configuration, the tree or the lists drops the graph.  Spherical basis,
+ * useRopt 0 and no MASS normalisation only (EXAFMM_HIP_EUNSUP otherwise:
+ * there the radii or the expansions depend on the charges).  No counterpart
+ * in the reference, whose driver rebuilds the tree for every evaluation
+ * (examples/serial.cxx:41-85); for iterative solvers whose iterate is the
+ * charge vector, and for repeated batched evaluations of fixed systems.
+ * _device: q is a device pointer. */
+int exafmm_hip_evaluate_charges(exafmm_hip_handle *h, int n, const double *q);
+int exafmm_hip_evaluate_charges_device(exafmm_hip_handle *h, int n, const double *d_q);
+
 int exafmm_hip_get_stats(exafmm_hip_handle *h, exafmm_hip_stats *s);
 
 /* Autotuning (no counterpart in the reference, whose parameters are fixed
