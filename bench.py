@@ -539,6 +539,14 @@ def main():
                 "tiles": s["num_m2l_tiles"]},
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
         "phases_ms": {k: round(v, 4) for k, v in med.items()},
+        # The same phases under the reference's logger timer names
+        # (bound_box.h:77, build_tree_omp.h, up_down_pass.h:120,136,
+        # traversal.h:242 -- its Traverse evaluates P2P and M2L --, serial.cxx:43).
+        "phases_ms_reference_names": {"Get bounds": round(med["t_bounds"], 4), "Build tree": round(med["t_build"], 4),
+                                      "Upward pass": round(med["t_upward"], 4),
+                                      "Traverse": round(med["t_traverse"], 4),
+                                      "Downward pass": round(med["t_downward"], 4),
+                                      "Total FMM": round(med["t_total"], 4)},
         "charges_only_replay": fp64_charges,
         "counts": {"cells": s["num_cells"], "p2p_cell_pairs": s["num_p2p_pairs"],
                    "m2l_cell_pairs": s["num_m2l_pairs"], "p2p_body_pairs": p2p_pairs},
