@@ -65,7 +65,7 @@ struct SrcWalker {
   int e, sbase, dbase, dcount;
 };
 
-constexpr int kFillMax = 256;  // largest stage (slots per fill)
+constexpr int kFillMax = 256;  // largest stage (slots per fill): four map slots per lane
 
 struct SrcDesc {
   int b0[64];
@@ -82,7 +82,7 @@ __device__ inline int fill_stage(SrcWalker &w, int e1, int lane, const int *__re
                                  const unsigned char *__restrict__ img, const int *__restrict__ cBody,
                                  const int *__restrict__ cNBody, const double4 *__restrict__ body, SrcDesc &sd,
                                  Put put, int skip = -1) {
-  static_assert(STAGE == kFillMax, "the slot map covers one full stage, four slots per lane");
+  static_assert(STAGE <= kFillMax && STAGE % 64 == 0, "the slot map covers a stage, four slots per lane");
   static_assert((STAGE / 64) % NB == 0, "whole batches of loads");
   __syncthreads();  // the previous stage (and descriptor batch) are consumed
   if (w.e >= w.dbase + w.dcount) {
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(64)
         // Fill: consecutive source leaves (or parts of one) up to kP2PStage
         // bodies, each shifted by its periodic image.
         bool imaged = false;
-        const int cnt = fill_stage<kP2PStage>(
+        const int cnt = fill_stage<kP2PStage, kP2PStage / 64>(
             w, e1, lane, src, img, cBody, cNBody, body, sd,
             [&](int k, const double4 &vb, int im) {
               double4 v = vb;

@@ -101,3 +101,53 @@ def test_mixed_direct_error_bound():
         f.close()
     for i in range(2):
         assert errs["mixed"][i] <= 1.02 * errs["fp64"][i] + 3e-7, errs
+
+
+def test_mixed_close_pairs_across_leaves():
+    """Close pairs in different leaves: 2,048 pairs of bodies straddle the
+    uniform tree's leaf faces 1e-6 leaf widths apart, among 16,384 uniform
+    bodies.  Their separation is far below one FP32 ulp of the neighbourhood's
+    extent, so the mixed P2P must add the positions' lo parts for them (an
+    unmasked stage: the partner sits in another leaf); without that the force
+    on those bodies would be off by ~10%.  Bar: the mixed form within 1e-6 /
+    5e-7 of the FP64 FMM, and its error against direct summation on the
+    straddling bodies within 1.02x of the FP64 form's plus 1e-6 (their forces
+    are dominated by the exact pair term, so both errors sit at the FP32 /
+    FP64 rounding of that term)."""
+    rng = np.random.default_rng(21)
+    n0, npair = 16384, 2048
+    x = unit_cube(n0, 22)
+    x[0, :3] = -1.0  # the bounding box is [-1, 1]^3
+    x[1, :3] = 1.0
+    n = n0 + 2 * npair
+    # Uniform build (build_tree_omp.h:216-231): L levels, cell size d (float)
+    # from R0 = 1.00001, keys measured from the lower bound -1.
+    L = 1 + int(np.log(n // 64) / np.log(2) / 3)
+    w = float(np.float32(2 * 1.00001 / (1 << L)))
+    face = -1.0 + rng.integers(1, (1 << L), npair) * w  # interior leaf faces along x
+    eps = 1e-6 * w
+    y = rng.uniform(-0.9, 0.9, (npair, 2))
+    q = rng.uniform(-0.5, 0.5, 2 * npair) / n0
+    a = np.column_stack([face - eps / 2, y, q[:npair]])
+    b = np.column_stack([face + eps / 2, y, q[npair:]])
+    bodies = np.concatenate([x, a, b])
+    bodies[:, 3] -= bodies[:, 3].mean()
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[gpu] mixed close pairs {prec}", flush=True)
+        f = X.FMM(P=10, theta=0.4, ncrit=64, precision=prec)
+        f.set_bodies(bodies)
+        f.evaluate()
+        out[prec] = f.results()
+        if prec == "fp64":
+            ref = f.direct(np.arange(n0, n, dtype=np.int32))
+        f.close()
+    r64, rmx = out["fp64"], out["mixed"]
+    assert np.all(np.isfinite(rmx))
+    assert rel_l2(rmx[:, 0], r64[:, 0]) < 1e-6
+    assert rel_l2(rmx[:, 1:], r64[:, 1:]) < 5e-7
+    for sl in (slice(0, 1), slice(1, 4)):
+        e64 = rel_l2(r64[n0:][:, sl], ref[:, sl])
+        emx = rel_l2(rmx[n0:][:, sl], ref[:, sl])
+        print(f"[gpu] close pairs {sl}: FP64 {e64:.3e} mixed {emx:.3e}", flush=True)
+        assert emx <= 1.02 * e64 + 1e-6, (sl, emx, e64)
