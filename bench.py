@@ -536,7 +536,8 @@ def main():
                    "ncrit": 64, "tree": "uniform Morton (build_tree_omp.h)", "precision": "fp64"},
         "p2p_ginter_per_s": p2p_pairs / (t_p2p * 1e-3) / 1e9,
         "m2l": {"pairs": s["num_m2l_pairs"], "lattice_pairs": n_lat, "offsets": s["num_m2l_offsets"],
-                "tiles": s["num_m2l_tiles"]},
+                "tiles": s["num_m2l_tiles"], "steps": s["num_m2l_steps"],
+                "busy_columns": round(n_lat / (64 * s["num_m2l_steps"]), 4) if s["num_m2l_steps"] else None},
         "rel_l2_err": {"pot": err_pot, "acc": err_acc, "targets": 128},
         "phases_ms": {k: round(v, 4) for k, v in med.items()},
         # The same phases under the reference's logger timer names

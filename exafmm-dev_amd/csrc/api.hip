@@ -73,6 +73,7 @@ static void do_build(Handle &h) {
     run_build_uniform(h);
   else
     run_build_adaptive(h);
+  h.key_lvloff = h.nsys == 1 && h.cfg.builder == 1;
   h.have_tree = true;
   h.have_up = h.have_trav = false;
   h.lat_ms_ready = false;  // scaled multipoles of an older tree
@@ -616,6 +617,7 @@ int exafmm_hip_get_stats(exafmm_hip_handle *H, exafmm_hip_stats *s) {
     h.stats.num_m2l_lattice = h.lat_stats_pairs;
     h.stats.num_m2l_offsets = h.lat_stats_noff;
     h.stats.num_m2l_tiles = h.lat_stats_tiles;
+    h.stats.num_m2l_steps = h.lat_stats_steps;
     *s = h.stats;
   });
 }

@@ -99,6 +99,10 @@ struct Handle {
   // Batched evaluation: nsys > 1 independent systems, system s owning bodies
   // [sys_off[s], sys_off[s+1]) of the input; the tree is a forest.
   int nsys = 1;
+  // cKey holds the Morton index on the cell's level grid plus the level
+  // offset (8^l - 1) / 7 (adaptive builder), or the index alone (uniform
+  // builder, forests).
+  bool key_lvloff = false;
   std::vector<int> sys_off;
   DBuf<int> sys_offd;
   DBuf<double> sys_par;  // per-system builder parameters
@@ -185,6 +189,7 @@ struct Handle {
   M2LList m2l_last{};                          // the pairs the last run_m2l gave the per-pair kernel
   int64_t lat_stats_pairs = 0;
   int lat_stats_noff = 0, lat_stats_tiles = 0;
+  int64_t lat_stats_steps = 0;
   bool lat_timed = false, pair_timed = false;  // ev[12..13]: MFMA kernel, ev[14..15]: per-pair kernel
 
   // M2L lane schedule (a pure function of P, the basis and the table type).

@@ -17,8 +17,10 @@
 // S1 = S'_{j+n}^{m-k}(d), S2 = S'_{j+n}^{-m-k}(d), zero unless j + n < P.
 //
 // Work split: targets are grouped into tiles of 64 cells of one level and
-// one octant (cells of one octant share most of their interaction offsets,
-// so a tile's columns are mostly busy at every offset);
+// one octant, ordered by their distances to the domain's faces (cells of one
+// octant share most of their interaction offsets, and those of one face
+// class also the missing ones, so a tile's columns are mostly busy at every
+// offset);
 // a tile's local expansions (112 rows x 64 columns) stay in MFMA
 // accumulators while the workgroup walks the offsets: per offset the matrix
 // T(d) is staged in LDS in MFMA fragment order (only the 16 x 4 blocks that
@@ -210,17 +212,47 @@ __global__ void __launch_bounds__(64) k_lat_tmat(const int *__restrict__ okey, R
   for (int i = S::NBLK * 64 + lane; i < TPad<P, R>::TEP; i += 64) T[(size_t)o * TPad<P, R>::TEP + i] = 0;
 }
 
-// Target tiles: cells with M2L pairs keyed by (level, octant); a radix sort
-// on the key keeps cell order inside a group.
+// Target tiles: cells with M2L pairs grouped by (level, octant), and inside
+// a group ordered by face class, then cell order (a radix sort keeps cell
+// order among equal keys).  The face class is the six distances of the cell
+// to the faces of its level's grid, each clipped at kTileK: cells near the
+// domain boundary lack the offsets that point outside it, and cells of one
+// class lack the same ones, so tiles cut from class-ordered cells have
+// fewer steps with idle columns.  The class only orders work: any order
+// gives the same pairs.
+constexpr int kTileK = 4;
+constexpr int kTileClassBits = 14;  // (kTileK + 1)^6 = 15625 classes
+static_assert((kTileK + 1) * (kTileK + 1) * (kTileK + 1) * (kTileK + 1) * (kTileK + 1) * (kTileK + 1) <=
+                  (1 << kTileClassBits),
+              "face classes fit their key bits");
+constexpr int kTileNoGroup = 255 << kTileClassBits;  // cells without pairs sort last
+
+// Face class of the cell whose Morton index on its level's grid is kk
+// (coordinate bits interleaved x, y, z from bit 0).
+__device__ inline int face_class(uint64_t kk, int l) {
+  int g[3] = {0, 0, 0};
+  for (int i = 0; i < l && i < 21; i++)
+    for (int a = 0; a < 3; a++) g[a] |= (int)((kk >> (3 * i + a)) & 1) << i;
+  const int64_t side = (int64_t)1 << (l < 21 ? l : 21);
+  int cls = 0;
+  for (int a = 0; a < 3; a++) {
+    const int lo = (int)min<int64_t>(g[a], kTileK), hi = (int)min<int64_t>(side - 1 - g[a], kTileK);
+    cls = (cls * (kTileK + 1) + lo) * (kTileK + 1) + hi;
+  }
+  return cls;
+}
+
 __global__ void k_tile_key(int nc, const int *__restrict__ has, const int *__restrict__ lvl,
-                           const uint64_t *__restrict__ ckey, int *__restrict__ key, int *__restrict__ val) {
+                           const uint64_t *__restrict__ ckey, bool key_lvloff, int *__restrict__ key,
+                           int *__restrict__ val) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  int k = 255;
+  int k = kTileNoGroup;
   const int l = lvl[c];
   if (has[c] && l > 0 && l < 31) {
-    const uint64_t lvloff = ((1ull << (3 * l)) - 1) / 7;
-    k = l * 8 + (int)((ckey[c] - lvloff) & 7);
+    const uint64_t lvloff = key_lvloff ? ((1ull << (3 * l)) - 1) / 7 : 0;
+    const uint64_t kk = ckey[c] - lvloff;
+    k = ((l * 8 + (int)(kk & 7)) << kTileClassBits) | face_class(kk, l);
   }
   key[c] = k;
   val[c] = c;
@@ -236,10 +268,10 @@ __global__ void k_mark_has(const int *__restrict__ tgt, int64_t np, int *__restr
 __global__ void k_tile_hist(int nc, const int *__restrict__ key, int *__restrict__ hist) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nc) return;
-  const int k = key[p];
+  const int k = key[p] >> kTileClassBits;
   if (k >= 255) return;
-  if (p == 0 || key[p - 1] != k) atomicSub(&hist[k], p);  // one start per group
-  if (p == nc - 1 || key[p + 1] != k) atomicAdd(&hist[k], p + 1);  // one end per group
+  if (p == 0 || key[p - 1] >> kTileClassBits != k) atomicSub(&hist[k], p);  // one start per group
+  if (p == nc - 1 || key[p + 1] >> kTileClassBits != k) atomicAdd(&hist[k], p + 1);  // one end per group
 }
 
 // gb[0..255] = group start in sorted order, gb[256..511] = first chunk of
@@ -250,7 +282,7 @@ __global__ void k_tile_assign(int nc, const int *__restrict__ skey, const int *_
                               int *__restrict__ tile_of, int *__restrict__ col_of) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nc) return;
-  const int g = skey[p], c = sval[p];
+  const int g = skey[p] >> kTileClassBits, c = sval[p];
   if (g >= 255) {
     tile_of[c] = -1;
     return;
@@ -410,8 +442,8 @@ __global__ void __launch_bounds__(256)
 // and keeps them in its L2.
 constexpr int kLatSeg = 8;
 // Words of Handle::hpin the lattice set-up reads back.
-constexpr int kPinGhist = 64, kPinNoff = 64 + 256, kPinNnew = 64 + 257;
-static_assert(kPinNnew < Handle::kPinWords, "pinned words");
+constexpr int kPinGhist = 64, kPinNoff = 64 + 256, kPinNnew = 64 + 257, kPinSteps = 64 + 258;
+static_assert(kPinSteps < Handle::kPinWords, "pinned words");
 
 typedef __attribute__((address_space(3))) void lds_void;
 // s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left alone.
@@ -424,11 +456,12 @@ typedef __attribute__((address_space(1))) void glb_void;
 // every step (a step's K-sum is the only FP32 accumulation).  Software
 // pipeline: while step i runs on the matrix cores, T of step i + 1 streams
 // into the other LDS buffer by LDS DMA (global_load_lds_dwordx4), the
-// multipole columns of step i + 1 stream into registers and the slot indices
-// of step i + 2 are read; one barrier per step.  The two LDS buffers are
-// separate arrays and the step loop is unrolled by two, so every LDS access
-// names its buffer statically.  The segment's partial expansions (unscaled)
-// go to part[] for k_lat_reduce.
+// multipole columns of step i + 1 stream into the other register set and the
+// slot indices of step i + 2 are read; one barrier per step.  The two LDS
+// buffers are separate arrays, the two column sets separate register arrays,
+// and the step loop is unrolled by two, so every access names its buffer
+// statically (no register copy between steps).  The segment's partial
+// expansions (unscaled) go to part[] for k_lat_reduce.
 template <int P, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 10 ? 2 : 1, 2)))
     k_m2l_mfma(int nc, int nitems, const int *__restrict__ step0, const int *__restrict__ alist,
@@ -465,7 +498,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
       for (int ch = wave; ch < NCH; ch += 4)
         __builtin_amdgcn_global_load_lds((glb_void *)(Tn + ch * CH), (lds_void *)(dst + ch * CH), 16, 0, 0);
     };
-    R b[KCP];
+    R b0[KCP], b1[KCP];
     int sn = -1;
     // Column of slot value sl (-2 - source cell, or -1: row nc of zeros)
     // into v[0 .. KCP): one 16-byte load per VL entries (its kq group of the
@@ -480,14 +513,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
         for (int u = 0; u < VL; u++) v[j * VL + u] = x[u];
       }
     };
-    // Step i from buffer CUR = (i - i0) & 1 while step i + 1 is fetched into
-    // the other one.
+    // Step i from buffers CUR = (i - i0) & 1 while step i + 1 is fetched
+    // into the other ones.
     auto step = [&](auto curc, int i) {
       constexpr int CUR = decltype(curc)::value;
       const R *sTc = CUR ? sT1 : sT0;
       R *sTn = CUR ? sT0 : sT1;
+      const R *b = CUR ? b1 : b0;
+      R *bn = CUR ? b0 : b1;
       const bool more = i + 1 < i1;
-      R bn[KCP];
       int snn = -1;
       if (more) {
         load_col(bn, sn);
@@ -508,11 +542,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
             acc[rb][r] = 0;
           }
       }
-      if (more) {
-#pragma unroll
-        for (int kc = 0; kc < KCP; kc++) b[kc] = bn[kc];
-        sn = snn;
-      }
+      if (more) sn = snn;
       __builtin_amdgcn_s_waitcnt(kVmcnt0);  // this wave's DMA and loads have landed
       __syncthreads();
     };
@@ -520,7 +550,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 1
     if (i0 < i1) {
       fetch_T(sT0, i0);
       sn = slot[(size_t)i0 * 64 + col];
-      load_col(b, sn);
+      load_col(b0, sn);
       sn = i0 + 1 < i1 ? slot[(size_t)(i0 + 1) * 64 + col] : -1;
     }
     __builtin_amdgcn_s_waitcnt(kVmcnt0);
@@ -683,7 +713,7 @@ static void lattice_begin(Handle &h) {
   exclusive_scan_int(h, h.lat_tc_flag.p, h.lat_tc_scan.p, kLatKeys + 1);
   hipLaunchKernelGGL(k_lat_number, dim3(div_up(kLatKeys, TPB)), dim3(TPB), 0, h.stream, h.lat_flag.p, h.lat_kscan.p,
                      kLatKeys, h.lat_ookey.p, h.lat_okey.p);
-  // 3. Target tiles of one level and one octant (cell order inside a tile).
+  // 3. Target tiles of one level and one octant (face class, then cell order).
   h.lat_tkey.resize(2 * (size_t)nc);
   h.lat_tval.resize(2 * (size_t)nc);
   h.lat_thist.resize(256);
@@ -692,9 +722,9 @@ static void lattice_begin(Handle &h) {
   hipLaunchKernelGGL(k_fill_int, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, h.lat_has.p, (int64_t)nc, 0);
   hipLaunchKernelGGL(k_mark_has, dim3(gp), dim3(TPB), 0, h.stream, ptgt, np, h.lat_has.p);
   hipLaunchKernelGGL(k_tile_key, dim3(div_up(nc, TPB)), dim3(TPB), 0, h.stream, nc, h.lat_has.p, h.cLevel.p, h.cKey.p,
-                     tk, tv);
+                     h.key_lvloff, tk, tv);
   bool alt = false;
-  sort_pairs_int_int(h, tk, tk + nc, tv, tv + nc, nc, 8, alt);
+  sort_pairs_int_int(h, tk, tk + nc, tv, tv + nc, nc, 8 + kTileClassBits, alt);
   h.lat_tile_alt = alt;
   const int *skey = alt ? tk + nc : tk;
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, h.stream, h.lat_thist.p, (int64_t)256, 0);
@@ -828,9 +858,13 @@ static void lattice_run(Handle &h, M2LList &irr) {
                      h.lat_pslot.p, h.lat_slot.p);
   hipLaunchKernelGGL(k_lat_check, dim3(gp), dim3(TPB), 0, h.stream, np, h.lat_pslot.p, psrc, h.lat_slot.p,
                      h.lat_irr.p, h.lat_cnt.p);
-  // The irregular count is read back behind the contraction's launch (pinned
-  // host word; the copy completes before the contraction starts).
+  // The irregular count (and the step count, for the statistics) is read
+  // back behind the contraction's launch (pinned host words; the copies
+  // complete before the contraction starts).
   EXA_HIP(hipMemcpyAsync(h.hpin, h.lat_cnt.p, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  if (nsteps_all)
+    EXA_HIP(hipMemcpyAsync(h.hpin + kPinSteps, h.lat_aidx.p + nsteps_all, sizeof(int), hipMemcpyDeviceToHost,
+                           h.stream));
   EXA_HIP(hipEventRecord(h.ev_cnt, h.stream));
   h.lat_stats_noff = noff;
   h.lat_stats_tiles = ntiles;
@@ -855,6 +889,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
   }
   EXA_HIP(hipEventSynchronize(h.ev_cnt));
   const int nirr = *h.hpin;
+  h.lat_stats_steps = nsteps_all ? h.hpin[kPinSteps] : 0;
   // 7. The irregular pairs (none for a uniform free-space tree), in
   // emission order, grouped by target (a stable sort: within a target the
   // order of the full list grouped by target), queued behind the contraction.
@@ -928,6 +963,7 @@ bool run_m2l_lattice(Handle &h, M2LList &irr) {
   h.lat_run_ntiles = 0;
   h.lat_stats_pairs = 0;
   h.lat_stats_noff = h.lat_stats_tiles = 0;
+  h.lat_stats_steps = 0;
   if (!lattice_applies(h) || h.nm2l == 0) {
     h.lat_ms_ready = false;
     return false;

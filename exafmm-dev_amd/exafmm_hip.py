@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
         ("num_p2p_pairs", ctypes.c_int64), ("num_m2l_pairs", ctypes.c_int64), ("num_p2p_bodies", ctypes.c_int64),
         ("num_cells", ctypes.c_int), ("num_levels", ctypes.c_int), ("num_m2l_lattice", ctypes.c_int64),
         ("num_m2l_offsets", ctypes.c_int), ("num_m2l_tiles", ctypes.c_int), ("t_m2l_mfma", ctypes.c_double),
-        ("t_m2l_pairs", ctypes.c_double)]
+        ("t_m2l_pairs", ctypes.c_double), ("num_m2l_steps", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

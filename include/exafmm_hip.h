@@ -71,6 +71,9 @@ typedef struct {
   int num_m2l_tiles;       /* 64-cell target tiles of the contraction          */
   double t_m2l_mfma;       /* ms of the MFMA contraction kernel alone           */
   double t_m2l_pairs;      /* ms of the per-pair M2L kernel alone               */
+  int64_t num_m2l_steps;   /* (tile, offset) steps of the contraction: one
+                              64-column MFMA pass each; num_m2l_lattice /
+                              (64 num_m2l_steps) = share of busy columns      */
 } exafmm_hip_stats;
 
 /* Create / destroy.  Replaces the constructors BoundBox(nspawn),
