@@ -18,6 +18,7 @@
 // by (node rank, octant), sorted stably and scattered back into the same
 // positions, which is exactly a stable per-node octant partition; bodies of
 // finished leaves never move.
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -315,7 +316,9 @@ void run_build_adaptive(Handle &h) {
   DBuf<double4> &nX = h.at_X;
   DBuf<int> &nstart = h.at_start, &ncount = h.at_count, &nparent = h.at_parent, &nleaf = h.at_leaf,
             &nlevel = h.at_level, &nfirst = h.at_first, &nnch = h.at_nchild;
-  size_t ncap = 0;
+  // The node arrays persist in the handle and grow together: what they hold
+  // from an earlier build is reused (no allocation, no synchronisation).
+  size_t ncap = std::min({nX.cap, nstart.cap, ncount.cap, nparent.cap, nleaf.cap, nlevel.cap, nfirst.cap, nnch.cap});
   auto grow = [&](auto &b, size_t want, size_t keep) {
     using Tt = typename std::remove_reference<decltype(*b.p)>::type;
     DBuf<Tt> nb;
