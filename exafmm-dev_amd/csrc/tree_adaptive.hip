@@ -120,7 +120,7 @@ struct ChildArgs {
   const int *skey;
   const int *apos;
   const int *hpos;        // sorted position of each child's first body
-  int nch;                // children created at this level
+  const int *nch;         // children created at this level (device word)
   int na;
   const double4 *nX;      // centres of the active (parent) nodes
   const int *ngid;        // global node id of each active node
@@ -135,9 +135,10 @@ struct ChildArgs {
 
 __global__ void k_children(ChildArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.nch) return;
+  const int nch = *a.nch;
+  if (c >= nch) return;
   const int i = a.hpos[c];
-  const int e = c + 1 < a.nch ? a.hpos[c + 1] : a.na;
+  const int e = c + 1 < nch ? a.hpos[c + 1] : a.na;
   const int k = a.skey[i];
   const int pr = k >> 3, oct = k & 7;
   const double4 P = a.nX[pr];
@@ -165,24 +166,39 @@ __global__ void k_identity(int *__restrict__ v, int m) {
   if (i < m) v[i] = i;
 }
 
-// Gather the next level's active nodes (children that are not leaves).
-__global__ void k_next_active(const int *__restrict__ leaf, int first, int cnt, const int *__restrict__ scan,
-                              const double4 *__restrict__ X, const int *__restrict__ start,
-                              const int *__restrict__ count, double4 *__restrict__ aX, int *__restrict__ astart,
-                              int *__restrict__ acount, int *__restrict__ agid) {
+// Gather the next level's active nodes (children that are not leaves) and
+// count their bodies into *nbody (zeroed beforehand).  *cnt: the level's
+// children (device word).
+__global__ void k_next_active(const int *__restrict__ leaf, int first, const int *__restrict__ cnt,
+                              const int *__restrict__ scan, const double4 *__restrict__ X,
+                              const int *__restrict__ start, const int *__restrict__ count, double4 *__restrict__ aX,
+                              int *__restrict__ astart, int *__restrict__ acount, int *__restrict__ agid,
+                              int *__restrict__ nbody) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cnt || leaf[first + i]) return;
+  if (i >= *cnt || leaf[first + i]) return;
   const int a = scan[i];
   aX[a] = X[first + i];
   astart[a] = start[first + i];
   acount[a] = count[first + i];
   agid[a] = first + i;
+  atomicAdd(nbody, count[first + i]);
 }
 
-__global__ void k_not_leaf(const int *__restrict__ leaf, int first, int cnt, int *__restrict__ out) {
+// out[i] = child i is not a leaf, for i < *cnt; 0 up to out[bound].
+__global__ void k_not_leaf(const int *__restrict__ leaf, int first, const int *__restrict__ cnt, int bound,
+                           int *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > cnt) return;
-  out[i] = i < cnt ? !leaf[first + i] : 0;
+  if (i > bound) return;
+  out[i] = i < *cnt ? !leaf[first + i] : 0;
+}
+
+// The level's three counts into w[0..2] for one read-back: children, next
+// active nodes, their bodies.
+__global__ void k_level_counts(const int *__restrict__ nch, const int *__restrict__ nnext,
+                               const int *__restrict__ nbody, int *__restrict__ w) {
+  w[0] = *nch;
+  w[1] = *nnext;
+  w[2] = *nbody;
 }
 
 // Subtree sizes, one level at a time from the deepest.
@@ -291,9 +307,9 @@ __global__ void k_validate(int nc, int n, const int *__restrict__ cParent, const
   if (!ok) atomicAdd(bad, 1);
 }
 
-__global__ void k_fill_level(int *__restrict__ level, int first, int cnt, int lev) {
+__global__ void k_fill_level(int *__restrict__ level, int first, const int *__restrict__ cnt, int lev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cnt) level[first + i] = lev;
+  if (i < *cnt) level[first + i] = lev;
 }
 
 }  // namespace
@@ -370,20 +386,22 @@ void run_build_adaptive(Handle &h) {
   levels.push_back({1, 0});
   int total = 1;
   int nact = n > ncrit ? 1 : 0;
+  int na = nact ? n : 0;  // bodies of the active nodes
   int lev = 0;
+  // Device words: [0] bodies of the next active set, [1..3] the level's
+  // counts gathered for one read-back per level (pinned words 16..18).
   h.cnt.reserve(4);
+  int *const hcnt = h.hpin + 16;
   while (nact > 0) {
     if (lev >= 20)
       throw Error(EXAFMM_HIP_EUNSUP, "adaptive tree deeper than 20 levels (more than ncrit coincident bodies?)");
-    // 1. Flag and compact the bodies of active nodes.
+    // 1. Flag and compact the bodies of active nodes (na of them, known
+    // from the previous level's read-back).
     h.ifl.reserve(n + 1);
     h.ic.reserve(n + 1);
     hipLaunchKernelGGL(k_active_flags, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, astart.p, acount.p, nact, n,
                        h.ifl.p);
     exclusive_scan_int(h, h.ifl.p, h.ic.p, n + 1);
-    int na = 0;
-    EXA_HIP(hipMemcpyAsync(&na, h.ic.p + n, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipStreamSynchronize(h.stream));
     h.ie.reserve(na + 1);  // apos
     h.k64a.reserve(na + 1);
     h.ia.reserve(na + 1);
@@ -410,22 +428,22 @@ void run_build_adaptive(Handle &h) {
     std::swap(h.body4.cap, b2.cap);
     std::swap(h.bidx.p, i2.p);
     std::swap(h.bidx.cap, i2.cap);
-    // 4. Children: one per (node, octant) run.
+    // 4. Children: one per (node, octant) run; their count stays on the
+    // device (hscan[na]) and launches are sized by its bound.
     h.id.reserve(3 * ((size_t)na + 1));
     int *head = h.id.p, *hscan = h.id.p + na + 1, *hpos = h.id.p + 2 * ((size_t)na + 1);
     hipLaunchKernelGGL(k_child_heads, dim3(div_up(na + 1, T)), dim3(T), 0, h.stream, skey, na, head);
     exclusive_scan_int(h, head, hscan, na + 1);
-    int nchildren = 0;
-    EXA_HIP(hipMemcpyAsync(&nchildren, hscan + na, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipStreamSynchronize(h.stream));
+    const int *nch_dev = hscan + na;
+    const int bound = (int)std::min<int64_t>(8LL * nact, na);  // children <= 8 per node, >= 1 body each
     hipLaunchKernelGGL(k_child_positions, dim3(div_up(na, T)), dim3(T), 0, h.stream, head, hscan, na, hpos);
-    reserve_nodes((size_t)total + nchildren + 1, (size_t)total);
+    reserve_nodes((size_t)total + bound + 1, (size_t)total);
     hipLaunchKernelGGL(k_reset_parents, dim3(div_up(nact, T)), dim3(T), 0, h.stream, agid.p, nact, nfirst.p, nnch.p);
     ChildArgs ca;
     ca.skey = skey;
     ca.apos = h.ie.p;
     ca.hpos = hpos;
-    ca.nch = nchildren;
+    ca.nch = nch_dev;
     ca.na = na;
     ca.nX = aX.p;
     ca.ngid = agid.p;
@@ -439,30 +457,35 @@ void run_build_adaptive(Handle &h) {
     ca.leaf = nleaf.p;
     ca.first_child = nfirst.p;
     ca.nchild = nnch.p;
-    hipLaunchKernelGGL(k_children, dim3(div_up(nchildren, T)), dim3(T), 0, h.stream, ca);
-    hipLaunchKernelGGL(k_fill_level, dim3(div_up(nchildren, T)), dim3(T), 0, h.stream, nlevel.p, total, nchildren,
+    hipLaunchKernelGGL(k_children, dim3(div_up(bound, T)), dim3(T), 0, h.stream, ca);
+    hipLaunchKernelGGL(k_fill_level, dim3(div_up(bound, T)), dim3(T), 0, h.stream, nlevel.p, total, nch_dev,
                        lev + 1);
-    levels.push_back({nchildren, total});
-    // 5. Next active set.
+    // 5. Next active set: the children that are not leaves, and their bodies.
     DBuf<int> &nlb = h.at_nl;
-    nlb.reserve(2 * ((size_t)nchildren + 1));
-    int *nl = nlb.p, *nls = nlb.p + nchildren + 1;
-    hipLaunchKernelGGL(k_not_leaf, dim3(div_up(nchildren + 1, T)), dim3(T), 0, h.stream, nleaf.p, total, nchildren,
+    nlb.reserve(2 * ((size_t)bound + 1));
+    int *nl = nlb.p, *nls = nlb.p + bound + 1;
+    hipLaunchKernelGGL(k_not_leaf, dim3(div_up(bound + 1, T)), dim3(T), 0, h.stream, nleaf.p, total, nch_dev, bound,
                        nl);
-    exclusive_scan_int(h, nl, nls, nchildren + 1);
-    int nnext = 0;
-    EXA_HIP(hipMemcpyAsync(&nnext, nls + nchildren, sizeof(int), hipMemcpyDeviceToHost, h.stream));
-    EXA_HIP(hipStreamSynchronize(h.stream));
-    if ((size_t)nnext + 1 > aX.cap) {  // the old active set is no longer needed
-      aX.reserve(nnext + 1);
-      astart.reserve(nnext + 1);
-      acount.reserve(nnext + 1);
-      agid.reserve(nnext + 1);
+    exclusive_scan_int(h, nl, nls, bound + 1);
+    if ((size_t)bound + 1 > aX.cap) {  // the current active set is read up to here: let it finish first
+      EXA_HIP(hipStreamSynchronize(h.stream));
+      aX.reserve(bound + 1);
+      astart.reserve(bound + 1);
+      acount.reserve(bound + 1);
+      agid.reserve(bound + 1);
     }
-    hipLaunchKernelGGL(k_next_active, dim3(div_up(nchildren, T)), dim3(T), 0, h.stream, nleaf.p, total, nchildren,
-                       nls, nX.p, nstart.p, ncount.p, aX.p, astart.p, acount.p, agid.p);
+    zero_device(h.cnt.p, 2 * sizeof(int), h.stream);
+    hipLaunchKernelGGL(k_next_active, dim3(div_up(bound, T)), dim3(T), 0, h.stream, nleaf.p, total, nch_dev, nls,
+                       nX.p, nstart.p, ncount.p, aX.p, astart.p, acount.p, agid.p, h.cnt.p);
+    // One read-back per level: children, next active nodes, their bodies.
+    hipLaunchKernelGGL(k_level_counts, dim3(1), dim3(1), 0, h.stream, nch_dev, nls + bound, h.cnt.p, h.cnt.p + 1);
+    EXA_HIP(hipMemcpyAsync(hcnt, h.cnt.p + 1, 3 * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    EXA_HIP(hipStreamSynchronize(h.stream));
+    const int nchildren = hcnt[0];
+    levels.push_back({nchildren, total});
     total += nchildren;
-    nact = nnext;
+    nact = hcnt[1];
+    na = hcnt[2];
     lev++;
   }
   // Subtree sizes (bottom-up), then cell numbering (top-down).
