@@ -33,64 +33,51 @@ struct NodeLevel {
   int first = 0;  // index of the level's first node in the global node arrays
 };
 
-__global__ void k_active_flags(const int *__restrict__ nstart, const int *__restrict__ ncount, int nact, int n,
-                               int *__restrict__ flag) {
-  // Mark bodies inside active nodes: node r covers [nstart[r], nstart[r] + ncount[r]).
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > n) return;
-  if (b == n) {
-    flag[b] = 0;
-    return;
-  }
-  int lo = 0, hi = nact;  // last node with start <= b
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (nstart[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  flag[b] = (nact > 0 && nstart[lo] <= b && b < nstart[lo] + ncount[lo]) ? 1 : 0;
-}
-
-__global__ void k_active_keys(const int *__restrict__ flag, const int *__restrict__ scan, int n,
-                              const int *__restrict__ nstart, const double4 *__restrict__ nX, int nact,
-                              const double4 *__restrict__ body, int *__restrict__ apos, int *__restrict__ key,
-                              int *__restrict__ val) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n || !flag[b]) return;
+// Active bodies in compacted order: position a belongs to active node r,
+// the last with aoff[r] <= a (aoff: exclusive scan of the active nodes' body
+// counts; a node's bodies are the contiguous range starting at astart[r]).
+// Key (node rank << 3 | octant) for the stable per-node octant partition.
+__global__ void k_active_keys(int na, const int *__restrict__ aoff, int nact, const int *__restrict__ astart,
+                              const double4 *__restrict__ nX, const double4 *__restrict__ body, int *__restrict__ apos,
+                              int *__restrict__ key, int *__restrict__ val) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
   int lo = 0, hi = nact;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (nstart[mid] <= b)
+    if (aoff[mid] <= a)
       lo = mid;
     else
       hi = mid;
   }
+  const int b = astart[lo] + (a - aoff[lo]);
   const double4 x = body[b], c = nX[lo];
   const int oct = (x.x > c.x) + ((x.y > c.y) << 1) + ((x.z > c.z) << 2);
-  const int a = scan[b];
   apos[a] = b;
   key[a] = (lo << 3) | oct;  // node rank < 2^28
   val[a] = a;
 }
 
-__global__ void k_active_scatter(const int *__restrict__ apos, const int *__restrict__ order, int na,
-                                 const double4 *__restrict__ bin, const int *__restrict__ iin,
-                                 double4 *__restrict__ bout, int *__restrict__ iout) {
+// The active bodies in sorted order into scratch, then back into the active
+// positions: a stable octant partition of every active node in place
+// (bodies of finished leaves are not touched).
+__global__ void k_gather_active(const int *__restrict__ apos, const int *__restrict__ order, int na,
+                                const double4 *__restrict__ bin, const int *__restrict__ iin, double4 *__restrict__ bt,
+                                int *__restrict__ it) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= na) return;
-  const int dst = apos[a], src = apos[order[a]];
-  bout[dst] = bin[src];
-  iout[dst] = iin[src];
+  const int src = apos[order[a]];
+  bt[a] = bin[src];
+  it[a] = iin[src];
 }
 
-__global__ void k_copy_inactive(const int *__restrict__ flag, int n, const double4 *__restrict__ bin,
-                                const int *__restrict__ iin, double4 *__restrict__ bout, int *__restrict__ iout) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n || flag[b]) return;
-  bout[b] = bin[b];
-  iout[b] = iin[b];
+__global__ void k_put_active(const int *__restrict__ apos, int na, const double4 *__restrict__ bt,
+                             const int *__restrict__ it, double4 *__restrict__ bout, int *__restrict__ iout) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int dst = apos[a];
+  bout[dst] = bt[a];
+  iout[dst] = it[a];
 }
 
 // Child heads: position a starts a child when its (node, octant) key differs
@@ -395,23 +382,20 @@ void run_build_adaptive(Handle &h) {
   while (nact > 0) {
     if (lev >= 20)
       throw Error(EXAFMM_HIP_EUNSUP, "adaptive tree deeper than 20 levels (more than ncrit coincident bodies?)");
-    // 1. Flag and compact the bodies of active nodes (na of them, known
-    // from the previous level's read-back).
-    h.ifl.reserve(n + 1);
-    h.ic.reserve(n + 1);
-    hipLaunchKernelGGL(k_active_flags, dim3(div_up(n + 1, T)), dim3(T), 0, h.stream, astart.p, acount.p, nact, n,
-                       h.ifl.p);
-    exclusive_scan_int(h, h.ifl.p, h.ic.p, n + 1);
+    // 1. The bodies of active nodes (na of them, known from the previous
+    // level's read-back) in compacted order, keyed by (node rank, octant).
+    h.ic.reserve((size_t)nact + 1);
+    exclusive_scan_int(h, acount.p, h.ic.p, nact);  // aoff
     h.ie.reserve(na + 1);  // apos
     h.k64a.reserve(na + 1);
     h.ia.reserve(na + 1);
     h.ib.reserve(na + 1);
-    // Keys (node rank << 3 | octant) as 32-bit ints: the two key buffers in
-    // k64a (a radix sort over 64-bit keys takes several times as many passes).
+    // Keys as 32-bit ints: the two key buffers in k64a (a radix sort over
+    // 64-bit keys takes several times as many passes).
     if (nact >= (1 << 28)) throw Error(EXAFMM_HIP_EUNSUP, "adaptive tree: too many nodes on one level");
     int *k32 = reinterpret_cast<int *>(h.k64a.p), *k32alt = k32 + (na + 1);
-    hipLaunchKernelGGL(k_active_keys, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ifl.p, h.ic.p, n, astart.p, aX.p,
-                       nact, h.body4.p, h.ie.p, k32, h.ia.p);
+    hipLaunchKernelGGL(k_active_keys, dim3(div_up(na, T)), dim3(T), 0, h.stream, na, h.ic.p, nact, astart.p, aX.p,
+                       h.body4.p, h.ie.p, k32, h.ia.p);
     // 2. Stable sort by (node rank, octant).
     int bits = 3;
     while ((1LL << (bits - 3)) < nact) bits++;
@@ -419,15 +403,11 @@ void run_build_adaptive(Handle &h) {
     sort_pairs_int_int(h, k32, k32alt, h.ia.p, h.ib.p, na, bits, alt);
     const int *skey = alt ? k32alt : k32;
     const int *order = alt ? h.ib.p : h.ia.p;
-    // 3. Scatter back into the same positions.
-    hipLaunchKernelGGL(k_copy_inactive, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.ifl.p, n, h.body4.p, h.bidx.p,
-                       b2.p, i2.p);
-    hipLaunchKernelGGL(k_active_scatter, dim3(div_up(na, T)), dim3(T), 0, h.stream, h.ie.p, order, na, h.body4.p,
+    // 3. Move the active bodies into sorted order, in place.
+    hipLaunchKernelGGL(k_gather_active, dim3(div_up(na, T)), dim3(T), 0, h.stream, h.ie.p, order, na, h.body4.p,
                        h.bidx.p, b2.p, i2.p);
-    std::swap(h.body4.p, b2.p);
-    std::swap(h.body4.cap, b2.cap);
-    std::swap(h.bidx.p, i2.p);
-    std::swap(h.bidx.cap, i2.cap);
+    hipLaunchKernelGGL(k_put_active, dim3(div_up(na, T)), dim3(T), 0, h.stream, h.ie.p, na, b2.p, i2.p, h.body4.p,
+                       h.bidx.p);
     // 4. Children: one per (node, octant) run; their count stays on the
     // device (hscan[na]) and launches are sized by its bound.
     h.id.reserve(3 * ((size_t)na + 1));
