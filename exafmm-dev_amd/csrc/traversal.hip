@@ -199,9 +199,8 @@ constexpr int kTravGrid = 512;
 
 struct TravState {
   long long nm, np;              // list lengths so far
-  long long base_m2l, base_p2p;  // offsets of the level being emitted
-  int nf, nf_next, fsel;         // frontier size, next size, current buffer (0: fa, 1: fb)
-  int emit, overflow, levels;
+  int nf, fsel;                  // frontier size, current buffer (0: fa, 1: fb)
+  int overflow, levels;
   int levels_small;              // levels done by k_trav_small
   int done_blocks;               // k_emit_dev blocks finished (the last one advances the level)
 };
@@ -216,11 +215,9 @@ __device__ inline void chunk_of(int nf, int &i0, int &i1) {
 }
 
 __global__ void k_trav_init(TravState *st, int nf) {
-  st->nm = st->np = st->base_m2l = st->base_p2p = 0;
+  st->nm = st->np = 0;
   st->nf = nf;
-  st->nf_next = 0;
   st->fsel = 0;
-  st->emit = 0;
   st->overflow = 0;
   st->levels = 0;
   st->levels_small = 0;
@@ -332,54 +329,45 @@ __global__ void __launch_bounds__(kTravBlock)
   if (threadIdx.x == 0) blk[blockIdx.x] = acc;
 }
 
-// One block: exclusive scan of the kTravGrid block totals, the level's
-// output offsets and the capacity check.
-__global__ void __launch_bounds__(kTravBlock)
-    k_trav_scan(const int4 *__restrict__ blk, int4 *__restrict__ blkofs, TravState *st, long long cap_m2l,
-                long long cap_p2p, int cap_f) {
-  constexpr int PER = kTravGrid / kTravBlock;
-  int4 v[PER];
-  int4 sum = make_int4(0, 0, 0, 0);
-  for (int k = 0; k < PER; k++) {
-    v[k] = blk[threadIdx.x * PER + k];
-    sum = add4(sum, v[k]);
-  }
-  int4 tot;
-  int4 run = block_exscan4(sum, &tot);
-  for (int k = 0; k < PER; k++) {
-    blkofs[threadIdx.x * PER + k] = run;
-    run = add4(run, v[k]);
-  }
-  if (threadIdx.x == 0) {
-    const bool ok = st->nm + tot.x <= cap_m2l && st->np + tot.y <= cap_p2p && tot.z <= cap_f;
-    st->base_m2l = st->nm;
-    st->base_p2p = st->np;
-    st->emit = ok && st->nf > 0;
-    if (!ok) st->overflow = 1;
-    if (ok) {
-      st->nm += tot.x;
-      st->np += tot.y;
-      st->nf_next = tot.z;
-    }
-  }
-}
-
+// Second pass of a level.  Every block first scans the kTravGrid block
+// totals of the first pass itself (8 KB, read by all blocks: no separate
+// scan launch), which gives its output offsets, the level's totals and the
+// capacity check (the same for every block); then it emits its chunk.  The
+// last block to finish moves the state to the next level.
 __global__ void __launch_bounds__(kTravBlock)
     k_emit_dev(CellView cv, int4 *__restrict__ fa, int4 *__restrict__ fb, TravState *st, int mutual,
-               const int *__restrict__ act, const int4 *__restrict__ blkofs, int *__restrict__ m2l_i,
-               int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
-               int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img) {
+               const int *__restrict__ act, const int4 *__restrict__ blk, long long cap_m2l, long long cap_p2p,
+               int cap_f, int *__restrict__ m2l_i, int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img,
+               int *__restrict__ p2p_i, int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img) {
+  constexpr int PER = kTravGrid / kTravBlock;
   __shared__ int s_last;
-  const int emit = st->emit;
+  __shared__ int4 s_run;
   const int nf = st->nf;
   const int fsel = st->fsel;
+  const long long bm = st->nm, bp = st->np;
   const int4 *fr = fsel ? fb : fa;
   int4 *next = fsel ? fa : fb;
-  const long long bm = st->base_m2l, bp = st->base_p2p;
+  int4 tot;
+  {
+    int4 v[PER];
+    int4 sum = make_int4(0, 0, 0, 0);
+    for (int k = 0; k < PER; k++) {
+      v[k] = blk[threadIdx.x * PER + k];
+      sum = add4(sum, v[k]);
+    }
+    int4 r = block_exscan4(sum, &tot);
+    for (int k = 0; k < PER; k++) {
+      if (threadIdx.x * PER + k == (int)blockIdx.x) s_run = r;
+      r = add4(r, v[k]);
+    }
+  }
+  __syncthreads();
+  const bool ok = bm + tot.x <= cap_m2l && bp + tot.y <= cap_p2p && tot.z <= cap_f;
+  const bool emit = ok && nf > 0;
   int i0, i1;
   chunk_of(nf, i0, i1);
   if (!emit) i1 = i0;
-  int4 run = blkofs[blockIdx.x];
+  int4 run = s_run;
   for (int s = i0; s < i1; s += kTravBlock) {
     const int i = s + threadIdx.x;
     int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
@@ -389,9 +377,9 @@ __global__ void __launch_bounds__(kTravBlock)
       a = act[i];
       c = pair_counts(cv, f, a, mutual);
     }
-    int4 tot;
-    const int4 ofs = add4(block_exscan4(c, &tot), run);
-    run = add4(run, tot);
+    int4 t4;
+    const int4 ofs = add4(block_exscan4(c, &t4), run);
+    run = add4(run, t4);
     if (i < i1) emit_pair(cv, f, a, mutual, ofs, bm, bp, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
   }
   // The last block to finish (every block has read the state by then) moves
@@ -401,10 +389,13 @@ __global__ void __launch_bounds__(kTravBlock)
   __syncthreads();
   if (s_last && threadIdx.x == 0) {
     st->done_blocks = 0;
+    if (!ok) st->overflow = 1;
     if (!emit) {
       st->nf = 0;
     } else {
-      st->nf = st->nf_next;
+      st->nm = bm + tot.x;
+      st->np = bp + tot.y;
+      st->nf = tot.z;
       st->fsel = fsel ^ 1;
       st->levels++;
     }
@@ -542,13 +533,12 @@ static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutua
   // (fa and fb were sized by the caller before the frontier was written.)
   h.ia.reserve(std::max(h.trav_fcap, nf0) + 1);
   h.ib.reserve(4 * (size_t)kTravGrid);
-  h.ic.reserve(4 * (size_t)kTravGrid);
   h.trav_state.reserve(sizeof(TravState));
   TravState *st = reinterpret_cast<TravState *>(h.trav_state.p);
   const long long cap_m = (long long)std::min(std::min(h.raw_mi.cap, h.raw_mj.cap), h.raw_mimg.cap) - 1;
   const long long cap_p = (long long)std::min(std::min(h.raw_pi.cap, h.raw_pj.cap), h.raw_pimg.cap) - 1;
   const int cap_fr = (int)std::min(std::min(h.fa.cap, h.fb.cap), h.ia.cap) - 1;  // frontier and action slots
-  int4 *blk = reinterpret_cast<int4 *>(h.ib.p), *blkofs = reinterpret_cast<int4 *>(h.ic.p);
+  int4 *blk = reinterpret_cast<int4 *>(h.ib.p);
   hipLaunchKernelGGL(k_trav_init, dim3(1), dim3(1), 0, h.stream, st, nf0);
   hipLaunchKernelGGL(k_trav_small, dim3(1), dim3(kSmallBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st, h.cfg.cycle,
                      h.cfg.nspawn, mutual, h.ia.p, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
@@ -560,10 +550,9 @@ static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutua
     for (int l = 0; l < batch; l++) {
       hipLaunchKernelGGL(k_classify_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
                          h.cfg.cycle, h.cfg.nspawn, mutual, h.ia.p, blk);
-      hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(kTravBlock), 0, h.stream, blk, blkofs, st, cap_m, cap_p, cap_fr);
       hipLaunchKernelGGL(k_emit_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
-                         mutual, h.ia.p, blkofs, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
-                         h.raw_pimg.p);
+                         mutual, h.ia.p, blk, cap_m, cap_p, cap_fr, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p,
+                         h.raw_pj.p, h.raw_pimg.p);
     }
     enq += batch;
     enqueue_multipoles(h);  // the second stream's work, while these levels run
