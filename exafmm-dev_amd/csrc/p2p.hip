@@ -200,6 +200,20 @@ struct TargetPasses {
   }
 };
 
+// FP64 kernel: pass A as above; pass B spreads its r targets over GB =
+// 64 / r groups (any whole number, not only powers of two), so at most r - 1
+// lanes idle: lane = g r + i holds target i of group g (lanes from r GB on
+// repeat group GB - 1 and are not summed).
+struct TargetPassesR {
+  int TA, r, GB;
+  __device__ explicit TargetPassesR(int n) {
+    TA = 8;
+    while (TA * 2 <= n) TA *= 2;
+    r = n > TA ? n - TA : 0;
+    GB = r > 0 ? 64 / r : 1;
+  }
+};
+
 __global__ void __launch_bounds__(64)
     k_p2p_f64(const int *__restrict__ tgt_list, int ntl_host, const int *__restrict__ ntl_dev,
               int *__restrict__ next, const int *__restrict__ off, const int *__restrict__ src,
@@ -221,14 +235,15 @@ __global__ void __launch_bounds__(64)
     const int c = tgt_list[bi];
     const int tb0 = cBody[c], tn = cNBody[c];
     const int e0 = off[c], e1 = off[c + 1];
-    // Sum the G group partials of the pass's targets in group order.
-    auto combine = [&](const double4 &acc, int T, int n, int tfirst) {
+    // Sum the G group partials (lanes gg T + i) of the pass's targets in
+    // group order.
+    auto combine = [&](const double4 &acc, int T, int G, int n, int tfirst) {
       __syncthreads();
       sacc[lane] = acc;
       __syncthreads();
       if (lane < n) {
         double4 s = sacc[lane];
-        for (int gg = 1; gg < 64 / T; gg++) {
+        for (int gg = 1; gg < G; gg++) {
           const double4 o = sacc[gg * T + lane];
           s.x += o.x;
           s.y += o.y;
@@ -245,11 +260,12 @@ __global__ void __launch_bounds__(64)
     };
     for (int tbase = 0; tbase < tn; tbase += 64) {
       const int nt = min(64, tn - tbase);
-      const TargetPasses tp(nt);
-      const int GA = 64 / tp.TA, GB = 64 / tp.TB;
-      const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
+      const TargetPassesR tp(nt);
+      const int GA = 64 / tp.TA, GB = tp.GB;
+      const int ga = lane / tp.TA, ia = lane % tp.TA;
+      const int rr = tp.r > 0 ? tp.r : 1, gb = min(lane / rr, GB - 1), ib = lane % rr;
       const double4 xa = ia < nt ? body[tb0 + tbase + ia] : make_double4(0, 0, 0, 0);
-      const double4 xb = ib < tp.r ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
+      const double4 xb = tp.r > 0 ? body[tb0 + tbase + tp.TA + ib] : make_double4(0, 0, 0, 0);
       double4 acca = make_double4(0, 0, 0, 0), accb = make_double4(0, 0, 0, 0);
       // One pass over a staged tile with G (1, 2, 4 or 8) a compile-time
       // constant and a wave-uniform trip count: the source addresses of a
@@ -301,17 +317,61 @@ __global__ void __launch_bounds__(64)
         else
           pass_g(mask, std::integral_constant<int, 8>{}, ncp, x, g, acc);
       };
-      // Pad a filled tile of cnt sources to a multiple of kP2PUnroll G (G
-      // of either pass, at most 8; the stage size is a multiple of 64) with
-      // zero charges far away, which add exactly zero, then run both passes.
+      // Pass B: G = GB groups at run time.  Whole steps of kP2PUnroll G
+      // sources run as in pass_g; the last, partial step reads only slots
+      // below ncp (a slot past it counts as a zero charge far away).
+      auto pass_b = [&](auto mask, int ncp) __attribute__((always_inline)) {
+        constexpr bool MASK = decltype(mask)::value;
+        const int G = __builtin_amdgcn_readfirstlane(GB), step = kP2PUnroll * G;
+        const int nfull = ncp / step * step;
+        const double4 *sg = sb + gb;
+        auto body_step = [&](int j, bool tail) __attribute__((always_inline)) {
+          double4 xj[kP2PUnroll];
+          double dx[kP2PUnroll], dy[kP2PUnroll], dz[kP2PUnroll], r2[kP2PUnroll], rs[kP2PUnroll];
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) {
+            const int k = j + u * G;
+            if (tail) {
+              xj[u] = sg[min(k, ncp - 1 - gb)];
+              if (gb + k >= ncp) xj[u] = make_double4(kFar, kFar, kFar, 0);
+            } else {
+              xj[u] = sg[k];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) {
+            dx[u] = xb.x - xj[u].x;
+            dy[u] = xb.y - xj[u].y;
+            dz[u] = xb.z - xj[u].z;
+            r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          }
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) rs[u] = rsqrt_pos(r2[u]);
+#pragma unroll
+          for (int u = 0; u < kP2PUnroll; u++) {
+            const double invr = MASK ? (r2[u] > 0 ? rs[u] : 0.0) : rs[u];
+            const double qinvr = xj[u].w * invr;
+            const double qinvr3 = qinvr * invr * invr;
+            accb.x += qinvr;
+            accb.y += dx[u] * qinvr3;
+            accb.z += dy[u] * qinvr3;
+            accb.w += dz[u] * qinvr3;
+          }
+        };
+        for (int j = 0; j < nfull; j += step) body_step(j, false);
+        if (nfull < ncp) body_step(nfull, true);
+      };
+      // Pad a filled tile of cnt sources to a multiple of kP2PUnroll GA
+      // (GA at most 8; the stage size is a multiple of 64) with zero charges
+      // far away, which add exactly zero, then run both passes.
       auto run = [&](auto mask, int cnt) __attribute__((always_inline)) {
-        const int g4 = kP2PUnroll * (tp.r > 0 ? GB : GA);
+        const int g4 = kP2PUnroll * GA;
         const int cntp = (cnt + g4 - 1) / g4 * g4;
         for (int k = cnt + lane; k < cntp; k += 64) sb[k] = make_double4(kFar, kFar, kFar, 0);
         __syncthreads();
         const int ncp = __builtin_amdgcn_readfirstlane(cntp);
         pass(mask, GA, ncp, xa, ga, acca);
-        if (tp.r > 0) pass(mask, GB, ncp, xb, gb, accb);
+        if (tp.r > 0) pass_b(mask, ncp);
       };
       // The target leaf's own bodies first (masked), then the rest of the
       // list with the own entry left out (the traversal lists every leaf
@@ -348,8 +408,8 @@ __global__ void __launch_bounds__(64)
         else
           run(std::false_type{}, cnt);
       }
-      combine(acca, tp.TA, min(nt, tp.TA), tbase);
-      if (tp.r > 0) combine(accb, tp.TB, tp.r, tbase + tp.TA);
+      combine(acca, tp.TA, GA, min(nt, tp.TA), tbase);
+      if (tp.r > 0) combine(accb, tp.r, GB, tp.r, tbase + tp.TA);
     }
   }
 }
