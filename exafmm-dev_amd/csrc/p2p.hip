@@ -10,12 +10,14 @@
 // traversal.h:333-337); the GPU accumulates the normalised quantities
 // directly.
 //
-// Lane layout: a leaf's targets (64 at a time) are split into two
-// power-of-two passes, TA = the largest power of two <= n (at least 8) and
-// TB = the remainder rounded up to a power of two (at least 8); in a pass of
-// T targets each target is replicated over G = 64 / T lane groups and group
-// g takes every G-th source body.  Both passes run over every staged source
-// tile, so 40 targets cost 32 + 8 lanes' worth of work instead of 64.
+// Lane layout: a leaf's targets (64 at a time) are split into two passes:
+// pass A takes TA = the largest power of two <= n (at least 8) targets, each
+// replicated over GA = 64 / TA lane groups; pass B takes the remaining r
+// targets over GB = 64 / r groups (any whole number), lane g r + i holding
+// target i of group g, so at most r - 1 lanes idle.  Group g takes every
+// G-th source body.  Both passes run over every staged source tile, so 37
+// targets cost 32 + 5 lanes' worth of work (64 / 12 groups) instead of 64.
+// (The mixed kernel keeps pass B at a power of two, at least 8 targets.)
 // Source bodies are staged through LDS (consecutive list entries, each
 // shifted by its periodic image); the G partial sums of a target are
 // combined in a fixed order at the end (deterministic, no atomics).
@@ -188,10 +190,13 @@ __device__ inline int next_leaf(int *next, int lane) {
 
 __global__ void k_zero_int(int *p) { *p = 0; }
 
-// The two target passes of a block of n <= 64 targets (see above).
-struct TargetPasses {
+// Mixed kernel: both passes of power-of-two sizes, TB = r rounded up to a
+// power of two (at least 8), so that every group stride is a compile-time
+// constant (a run-time stride cost the mixed kernel ~20 more VGPRs, which
+// crowded out the lattice set-up kernels running beside it).
+struct TargetPasses2 {
   int TA, TB, r;  // r = targets of pass B (0: no pass B)
-  __device__ explicit TargetPasses(int n) {
+  __device__ explicit TargetPasses2(int n) {
     TA = 8;
     while (TA * 2 <= n) TA *= 2;
     r = n > TA ? n - TA : 0;
@@ -200,13 +205,10 @@ struct TargetPasses {
   }
 };
 
-// FP64 kernel: pass A as above; pass B spreads its r targets over GB =
-// 64 / r groups (any whole number, not only powers of two), so at most r - 1
-// lanes idle: lane = g r + i holds target i of group g (lanes from r GB on
-// repeat group GB - 1 and are not summed).
-struct TargetPassesR {
+// The two target passes of a block of n <= 64 targets (see above).
+struct TargetPasses {
   int TA, r, GB;
-  __device__ explicit TargetPassesR(int n) {
+  __device__ explicit TargetPasses(int n) {
     TA = 8;
     while (TA * 2 <= n) TA *= 2;
     r = n > TA ? n - TA : 0;
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(64)
     };
     for (int tbase = 0; tbase < tn; tbase += 64) {
       const int nt = min(64, tn - tbase);
-      const TargetPassesR tp(nt);
+      const TargetPasses tp(nt);
       const int GA = 64 / tp.TA, GB = tp.GB;
       const int ga = lane / tp.TA, ia = lane % tp.TA;
       const int rr = tp.r > 0 ? tp.r : 1, gb = min(lane / rr, GB - 1), ib = lane % rr;
@@ -738,14 +740,14 @@ __global__ void __launch_bounds__(64)
     };
     for (int tbase = 0; tbase < tn; tbase += 64) {
       const int nt = min(64, tn - tbase);
-      const TargetPasses tp(nt);
+      const TargetPasses2 tp(nt);
       const int GA = 64 / tp.TA, GB = 64 / tp.TB;
       const int ga = lane / tp.TA, ia = lane % tp.TA, gb = lane / tp.TB, ib = lane % tp.TB;
       const Tgt ta = load_tgt(ia < nt, tb0 + tbase + ia);
       const Tgt tb = load_tgt(ib < tp.r, tb0 + tbase + tp.TA + ib);
       Acc acca, accb;
       // Passes with G (1, 2, 4 or 8) a compile-time constant and a
-      // wave-uniform trip count, as in k_p2p_f64.
+      // wave-uniform trip count, as pass A of k_p2p_f64.
       auto pass_g = [&](auto mask, auto gc, int ncp, const Tgt &t, int g,
                         Acc &a) __attribute__((always_inline)) {
         constexpr int G = decltype(gc)::value;
