@@ -487,12 +487,13 @@ def main():
             if tk > 0:
                 roofline["kernels"][k] = dict(rate(k, work, tk, bound, peak),
                                               traffic=measured_traffic(k) if n == 1 << 20 and P == 10 else None)
-    # The same two kernels with the CUs to themselves: P2P as one wave per
-    # leaf (EXAFMM_HIP_P2P_WAVES=0) fills the GPU, so the contraction starts
-    # after it; 3 evaluations after the timed ones, outside the timed region.
+    # The same two kernels with the GPU to themselves: with EXAFMM_HIP_SERIAL=1
+    # the contraction waits for P2P to finish (P2P keeps its grid; only the
+    # light lattice set-up runs beside it); 3 evaluations after the timed
+    # ones, outside the timed region.
     if args.isolated_rates:
-        old_env = os.environ.get("EXAFMM_HIP_P2P_WAVES")
-        os.environ["EXAFMM_HIP_P2P_WAVES"] = "0"
+        old_env = os.environ.get("EXAFMM_HIP_SERIAL")
+        os.environ["EXAFMM_HIP_SERIAL"] = "1"
         try:
             f = exafmm_hip.FMM(P=P, theta=0.4, ncrit=64, nspawn=1000, useRmax=1, useRopt=0, builder="uniform")
             f.set_bodies(x)
@@ -505,9 +506,9 @@ def main():
             f.close()
         finally:
             if old_env is None:
-                del os.environ["EXAFMM_HIP_P2P_WAVES"]
+                del os.environ["EXAFMM_HIP_SERIAL"]
             else:
-                os.environ["EXAFMM_HIP_P2P_WAVES"] = old_env
+                os.environ["EXAFMM_HIP_SERIAL"] = old_env
         iso_t = {"k_m2l_mfma<%d,double>" % P: float(np.median([d["t_m2l_mfma"] for d in iso])),
                  p2p_name: float(np.median([d["t_p2p"] for d in iso]))}
         roofline["isolated"] = {}

@@ -158,6 +158,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     if (k == 1 || (k >= 3 && k <= 5)) H->h.p2p_kmax = k;
   }
   if (const char *m = std::getenv("EXAFMM_HIP_LAT_WG")) H->h.lat_wg_cu = std::atoi(m);
+  if (const char *m = std::getenv("EXAFMM_HIP_SERIAL")) H->h.serial_heavy = std::atoi(m) != 0;
   // Rows of the lattice T(d) table kept across evaluations (16384 rows are
   // 0.5 GB in FP64 at P = 10).
   if (const char *m = std::getenv("EXAFMM_HIP_TCACHE_ROWS")) H->h.lat_tc_max = std::max(1, std::atoi(m));

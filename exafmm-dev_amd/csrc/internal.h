@@ -160,6 +160,9 @@ struct Handle {
   // workgroups per CU looping over (tile, segment) items (0: one workgroup
   // per item).  ncu = compute units.
   int p2p_wpc = 12, lat_wg_cu = 0, ncu = 0;
+  // Measurement switch: the contraction waits for P2P to finish, so each of
+  // the two runs with the GPU to itself (EXAFMM_HIP_SERIAL=1).
+  bool serial_heavy = false;
   // P2P targets per lane: 1 = one target per lane (k_p2p_f64 / k_p2p_mixed),
   // 3, 4 or 5 = the most targets per lane of the register-blocked kernels
   // (k_p2p_f64_rb / k_p2p_mixed_rb).

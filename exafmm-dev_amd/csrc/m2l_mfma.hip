@@ -881,6 +881,7 @@ static void lattice_run(Handle &h, M2LList &irr) {
     if (h.lat_wg_cu > 0) ngrid = std::min(ngrid, std::max(kLatSeg, (h.lat_wg_cu * h.ncu) / kLatSeg * kLatSeg));
     h.lat_run_ntiles = ntiles;
     h.lat_run_ngrid = ngrid;
+    if (h.serial_heavy) EXA_HIP(hipStreamWaitEvent(h.stream, h.ev[10], 0));  // P2P's end (second stream)
     EXA_HIP(hipEventRecord(h.ev[12], h.stream));
     h.lat_timed = true;
     lattice_contract<P>(h);
