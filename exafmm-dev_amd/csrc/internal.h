@@ -208,7 +208,7 @@ struct Handle {
   DBuf<unsigned char> tmp2;
   DBuf<uint64_t> k64a, k64b;
   DBuf<int> ia, ib, ic, id, ie, ifl;
-  DBuf<int4> fa, fb;               // traversal frontiers (ci, cj, img, pad)
+  DBuf<int2> fa, fb;               // traversal frontiers (ci, cj | img << 27)
   // Device-driven traversal: its state, the frontier capacity and the level
   // count of the last traversal (0: run the per-level passes).
   DBuf<char> trav_state;

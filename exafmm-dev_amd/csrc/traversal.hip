@@ -27,10 +27,30 @@ __device__ inline void image_offset(int img, double cycle, double &px, double &p
   pz = (img % 3 - 1) * cycle;
 }
 
+// A frontier pair packed in 8 bytes: (Ci, Cj | image << 27); the host keeps
+// cell indices below 2^27.
+constexpr int kImgShift = 27;
+__device__ inline int2 pack_pair(int ci, int cj, int img) { return make_int2(ci, cj | (img << kImgShift)); }
+__device__ inline int4 unpack_pair(int2 p) {
+  return make_int4(p.x, p.y & ((1 << kImgShift) - 1), (int)((unsigned)p.y >> kImgShift), 0);
+}
+
+// What the rule reads of the two cells of a pair, loaded up front (so that
+// the loads of several pairs are in flight at once).
+struct PairCells {
+  double4 Xi, Xj;
+  int nci, ncj, nbi, nbj;
+};
+
+__device__ inline PairCells load_cells(const CellView &cv, int ci, int cj) {
+  return PairCells{cv.X[ci], cv.X[cj], cv.nchild[ci], cv.nchild[cj], cv.nbody[ci], cv.nbody[cj]};
+}
+
 // action: 0 M2L, 1 P2P, 2 split Ci, 3 split Cj, 4 split both.
-__device__ inline int classify(const CellView &cv, int ci, int cj, int img, double cycle, int nspawn, int mutual) {
+__device__ inline int classify_cells(const PairCells &pc, int ci, int cj, int img, double cycle, int nspawn,
+                                     int mutual) {
 #pragma clang fp contract(off)
-  const double4 Xi = cv.X[ci], Xj = cv.X[cj];
+  const double4 Xi = pc.Xi, Xj = pc.Xj;
   double px, py, pz;
   image_offset(img, cycle, px, py, pz);
   const double dx = Xi.x - Xj.x - px;
@@ -42,31 +62,40 @@ __device__ inline int classify(const CellView &cv, int ci, int cj, int img, doub
   R2 += dz * dz;
   const double Rs = Xi.w + Xj.w;
   if (R2 > Rs * Rs) return 0;
-  const int li = cv.nchild[ci] == 0, lj = cv.nchild[cj] == 0;
-  if (li && lj) return cv.nbody[cj] == 0 ? 0 : 1;
+  const int li = pc.nci == 0, lj = pc.ncj == 0;
+  if (li && lj) return pc.nbj == 0 ? 0 : 1;
   if (lj) return 2;
   if (li) return 3;
-  if (cv.nbody[ci] + cv.nbody[cj] >= nspawn || (mutual && ci == cj)) return 4;
+  if (pc.nbi + pc.nbj >= nspawn || (mutual && ci == cj)) return 4;
   return Xi.w >= Xj.w ? 2 : 3;
+}
+
+__device__ inline int classify(const CellView &cv, int ci, int cj, int img, double cycle, int nspawn, int mutual) {
+  return classify_cells(load_cells(cv, ci, cj), ci, cj, img, cycle, nspawn, mutual);
 }
 
 // With mutual (the reference's -m 1), a visited pair (Ci, Cj), Ci != Cj,
 // stands for both directions: M2L and P2P emit both directed entries, and a
 // cell split against itself visits only the child pairs (a, b), a <= b, as
 // TraverseRange skips the (latter, former) quadrant (traversal.h:108-120).
-// Output counts of one frontier pair: (M2L entries, P2P entries, next pairs).
-__device__ inline int4 pair_counts(const CellView &cv, int4 f, int a, int mutual) {
+// Output counts of one frontier pair: (M2L entries, P2P entries, next pairs);
+// nci, ncj: the child counts of its two cells.
+__device__ inline int4 pair_counts_n(int4 f, int a, int mutual, int nci, int ncj) {
   const int both = (mutual && f.x != f.y) ? 2 : 1;
   int nx = 0;
   if (a == 2) {
-    nx = cv.nchild[f.x];
+    nx = nci;
   } else if (a == 3) {
-    nx = cv.nchild[f.y];
+    nx = ncj;
   } else if (a == 4) {
-    const int nci = cv.nchild[f.x];
-    nx = (mutual && f.x == f.y) ? nci * (nci + 1) / 2 : nci * cv.nchild[f.y];
+    nx = (mutual && f.x == f.y) ? nci * (nci + 1) / 2 : nci * ncj;
   }
   return make_int4(a == 0 ? both : 0, a == 1 ? both : 0, nx, 0);
+}
+
+__device__ inline int4 pair_counts(const CellView &cv, int4 f, int a, int mutual) {
+  if (a < 2) return pair_counts_n(f, a, mutual, 0, 0);
+  return pair_counts_n(f, a, mutual, cv.nchild[f.x], cv.nchild[f.y]);
 }
 
 __device__ inline int4 add4(int4 a, int4 b) { return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
@@ -87,9 +116,23 @@ __device__ inline int4 block_exscan4(int4 v, int4 *total) {
   if (lane == 63) wsum[w] = inc;
   __syncthreads();
   int4 base = make_int4(0, 0, 0, 0), tot = make_int4(0, 0, 0, 0);
-  for (int k = 0; k < NT / 64; k++) {
-    if (k < w) base = add4(base, wsum[k]);
-    tot = add4(tot, wsum[k]);
+  if (NT <= 256) {
+    for (int k = 0; k < NT / 64; k++) {
+      if (k < w) base = add4(base, wsum[k]);
+      tot = add4(tot, wsum[k]);
+    }
+  } else {  // many waves: the first wave scans the wave totals in place
+    if (w == 0) {
+      int4 x = lane < NT / 64 ? wsum[lane] : make_int4(0, 0, 0, 0);
+      for (int d = 1; d < NT / 64; d <<= 1) {
+        const int4 t = make_int4(__shfl_up(x.x, d), __shfl_up(x.y, d), __shfl_up(x.z, d), 0);
+        if (lane >= d) x = add4(x, t);
+      }
+      if (lane < NT / 64) wsum[lane] = x;
+    }
+    __syncthreads();
+    if (w > 0) base = wsum[w - 1];
+    tot = wsum[NT / 64 - 1];
   }
   __syncthreads();
   *total = tot;
@@ -98,14 +141,14 @@ __device__ inline int4 block_exscan4(int4 v, int4 *total) {
 
 // Pass 1 over the frontier: action per pair and the block's output totals.
 __global__ void __launch_bounds__(kTravBlock)
-    k_classify(CellView cv, const int4 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
-               int *__restrict__ act, int4 *__restrict__ blk) {
+    k_classify(CellView cv, const int2 *__restrict__ fr, int nf, double cycle, int nspawn, int mutual,
+               unsigned char *__restrict__ act, int4 *__restrict__ blk) {
   const int i = blockIdx.x * kTravBlock + threadIdx.x;
   int4 c = make_int4(0, 0, 0, 0);
   if (i < nf) {
-    const int4 f = fr[i];
+    const int4 f = unpack_pair(fr[i]);
     const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
-    act[i] = a;
+    act[i] = (unsigned char)a;
     c = pair_counts(cv, f, a, mutual);
   }
   int4 tot;
@@ -116,11 +159,12 @@ __global__ void __launch_bounds__(kTravBlock)
 
 // The outputs of frontier pair f (action a) at offsets ofs (M2L, P2P, next
 // frontier) past the level's list bases.
-__device__ inline void emit_pair(const CellView &cv, int4 f, int a, int mutual, int4 ofs, int64_t base_m2l,
-                                 int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
-                                 unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
-                                 int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img,
-                                 int4 *__restrict__ next) {
+// lk: (first child, child count) of Ci, then of Cj (read for splits only).
+__device__ inline void emit_pair_n(int4 f, int a, int4 lk, int mutual, int4 ofs, int64_t base_m2l,
+                                   int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                                   unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+                                   int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img,
+                                   int2 *__restrict__ next) {
   const bool both = mutual && f.x != f.y;
   if (a == 0) {
     const int64_t o = base_m2l + ofs.x;
@@ -144,39 +188,49 @@ __device__ inline void emit_pair(const CellView &cv, int4 f, int a, int mutual, 
     }
   } else {
     int o = ofs.z;
+    const int ci0 = lk.x, nci = lk.y, cj0 = lk.z, ncj = lk.w;
     if (a == 2) {
-      const int c0 = cv.child[f.x], nc = cv.nchild[f.x];
-      for (int c = 0; c < nc; c++) next[o++] = make_int4(c0 + c, f.y, f.z, 0);
+      for (int c = 0; c < nci; c++) next[o++] = pack_pair(ci0 + c, f.y, f.z);
     } else if (a == 3) {
-      const int c0 = cv.child[f.y], nc = cv.nchild[f.y];
-      for (int c = 0; c < nc; c++) next[o++] = make_int4(f.x, c0 + c, f.z, 0);
+      for (int c = 0; c < ncj; c++) next[o++] = pack_pair(f.x, cj0 + c, f.z);
     } else {
-      const int ci0 = cv.child[f.x], nci = cv.nchild[f.x];
-      const int cj0 = cv.child[f.y], ncj = cv.nchild[f.y];
       if (mutual && f.x == f.y) {
         for (int a1 = 0; a1 < nci; a1++)
-          for (int b1 = a1; b1 < nci; b1++) next[o++] = make_int4(ci0 + a1, ci0 + b1, f.z, 0);
+          for (int b1 = a1; b1 < nci; b1++) next[o++] = pack_pair(ci0 + a1, ci0 + b1, f.z);
       } else {
         for (int a1 = 0; a1 < nci; a1++)
-          for (int b1 = 0; b1 < ncj; b1++) next[o++] = make_int4(ci0 + a1, cj0 + b1, f.z, 0);
+          for (int b1 = 0; b1 < ncj; b1++) next[o++] = pack_pair(ci0 + a1, cj0 + b1, f.z);
       }
     }
   }
+}
+
+__device__ inline int4 load_links(const CellView &cv, int4 f) {
+  return make_int4(cv.child[f.x], cv.nchild[f.x], cv.child[f.y], cv.nchild[f.y]);
+}
+
+__device__ inline void emit_pair(const CellView &cv, int4 f, int a, int mutual, int4 ofs, int64_t base_m2l,
+                                 int64_t base_p2p, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+                                 unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+                                 int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img,
+                                 int2 *__restrict__ next) {
+  const int4 lk = a >= 2 ? load_links(cv, f) : make_int4(0, 0, 0, 0);
+  emit_pair_n(f, a, lk, mutual, ofs, base_m2l, base_p2p, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
 }
 
 // Pass 2: each pair's output offsets = its block's offset (scan of the block
 // totals) + its prefix inside the block; the same order as one global
 // exclusive scan, so the lists are deterministic.
 __global__ void __launch_bounds__(kTravBlock)
-    k_emit(CellView cv, const int4 *__restrict__ fr, int nf, int mutual, const int *__restrict__ act,
+    k_emit(CellView cv, const int2 *__restrict__ fr, int nf, int mutual, const unsigned char *__restrict__ act,
            const int4 *__restrict__ blkofs, int64_t base_m2l, int64_t base_p2p, int *__restrict__ m2l_i,
            int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
-           int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, int4 *__restrict__ next) {
+           int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, int2 *__restrict__ next) {
   const int i = blockIdx.x * kTravBlock + threadIdx.x;
   int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
   int a = -1;
   if (i < nf) {
-    f = fr[i];
+    f = unpack_pair(fr[i]);
     a = act[i];
     c = pair_counts(cv, f, a, mutual);
   }
@@ -191,7 +245,7 @@ __global__ void __launch_bounds__(kTravBlock)
 // lengths and which frontier buffer is current live in a TravState on the
 // device, so the host enqueues several levels without waiting for any of
 // them.  A fixed grid of kTravGrid blocks takes contiguous chunks of the
-// frontier (multiples of 256 pairs, in order), so the emission order -- and
+// frontier (multiples of 64 pairs, in order), so the emission order -- and
 // the lists -- are those of the per-level passes above.  Outputs that would
 // not fit the buffers set `overflow` and stop the traversal (the host then
 // runs the per-level passes, which grow the buffers).
@@ -208,8 +262,20 @@ struct TravState {
 constexpr int kSmallFrontier = 512;  // levels up to this size run in one workgroup
 constexpr int kSmallBlock = 1024;
 
+// The device-driven passes run NT threads per block, each taking K
+// consecutive frontier pairs per step (pairs s + K * t + (0 .. K - 1): in
+// order, so the block scan of the per-thread totals still gives the global
+// order) and issuing all their loads before it uses any.  Chunks are whole
+// multiples of 64 pairs (a step may end early), so that a small level with
+// many children per pair still spreads over many blocks.  Measured on the
+// 1M-body cube (six levels, up to 6 M pairs): classification with 1024
+// threads and one pair each, emission with 512 threads and two pairs each
+// (the fastest of 256/512/1024 threads with 1, 2 or 4 pairs).
+constexpr int kClassifyBlock = 1024;
+constexpr int kEmitBlock = 512, kEmitItems = 2;
+
 __device__ inline void chunk_of(int nf, int &i0, int &i1) {
-  const int c = ((nf + kTravGrid - 1) / kTravGrid + kTravBlock - 1) / kTravBlock * kTravBlock;
+  const int c = ((nf + kTravGrid - 1) / kTravGrid + 63) / 64 * 64;
   i0 = min(nf, (int)blockIdx.x * c);
   i1 = min(nf, i0 + c);
 }
@@ -228,10 +294,11 @@ __global__ void k_trav_init(TravState *st, int nf) {
 // passes of a level back to back (the frontier in 1024-pair chunks, in
 // order), no launch per level.
 __global__ void __launch_bounds__(kSmallBlock)
-    k_trav_small(CellView cv, int4 *__restrict__ fa, int4 *__restrict__ fb, TravState *st, double cycle,
-                 int nspawn, int mutual, int *__restrict__ act, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
-                 unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
-                 unsigned char *__restrict__ p2p_img, long long cap_m2l, long long cap_p2p, int cap_f) {
+    k_trav_small(CellView cv, int2 *__restrict__ fa, int2 *__restrict__ fb, TravState *st, double cycle,
+                 int nspawn, int mutual, unsigned char *__restrict__ act, int *__restrict__ m2l_i,
+                 int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i,
+                 int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img, long long cap_m2l, long long cap_p2p,
+                 int cap_f) {
   __shared__ int s_nf, s_fsel;
   __shared__ long long s_nm, s_np;
   if (threadIdx.x == 0) {
@@ -245,17 +312,17 @@ __global__ void __launch_bounds__(kSmallBlock)
   while (true) {
     const int nf = s_nf;
     if (nf == 0 || nf > kSmallFrontier) break;
-    const int4 *fr = s_fsel ? fb : fa;
-    int4 *next = s_fsel ? fa : fb;
+    const int2 *fr = s_fsel ? fb : fa;
+    int2 *next = s_fsel ? fa : fb;
     const long long nm = s_nm, np = s_np;
     int4 acc = make_int4(0, 0, 0, 0);
     for (int s0 = 0; s0 < nf; s0 += kSmallBlock) {
       const int i = s0 + threadIdx.x;
       int4 c = make_int4(0, 0, 0, 0);
       if (i < nf) {
-        const int4 f = fr[i];
+        const int4 f = unpack_pair(fr[i]);
         const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
-        act[i] = a;
+        act[i] = (unsigned char)a;
         c = pair_counts(cv, f, a, mutual);
       }
       int4 tot;
@@ -272,7 +339,7 @@ __global__ void __launch_bounds__(kSmallBlock)
       int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
       int a = -1;
       if (i < nf) {
-        f = fr[i];
+        f = unpack_pair(fr[i]);
         a = act[i];
         c = pair_counts(cv, f, a, mutual);
       }
@@ -305,28 +372,35 @@ __global__ void __launch_bounds__(kSmallBlock)
   }
 }
 
-__global__ void __launch_bounds__(kTravBlock)
-    k_classify_dev(CellView cv, const int4 *__restrict__ fa, const int4 *__restrict__ fb, const TravState *st,
-                   double cycle, int nspawn, int mutual, int *__restrict__ act, int4 *__restrict__ blk) {
+template <int NT, int K>
+__global__ void __launch_bounds__(NT, NT / 128)  // two blocks per CU
+    k_classify_dev(CellView cv, const int2 *__restrict__ fa, const int2 *__restrict__ fb, const TravState *st,
+                   double cycle, int nspawn, int mutual, unsigned char *__restrict__ act, int4 *__restrict__ blk) {
   const int nf = st->nf;
-  const int4 *fr = st->fsel ? fb : fa;
+  const int2 *fr = st->fsel ? fb : fa;
   int i0, i1;
   chunk_of(nf, i0, i1);
   int4 acc = make_int4(0, 0, 0, 0);
-  for (int s = i0; s < i1; s += kTravBlock) {
-    const int i = s + threadIdx.x;
-    int4 c = make_int4(0, 0, 0, 0);
-    if (i < i1) {
-      const int4 f = fr[i];
-      const int a = classify(cv, f.x, f.y, f.z, cycle, nspawn, mutual);
-      act[i] = a;
-      c = pair_counts(cv, f, a, mutual);
+  for (int s = i0; s < i1; s += NT * K) {
+    const int ib = s + threadIdx.x * K;
+    int4 f[K];
+    PairCells pc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) f[k] = ib + k < i1 ? unpack_pair(fr[ib + k]) : make_int4(0, 0, 13, 0);
+#pragma unroll
+    for (int k = 0; k < K; k++) pc[k] = load_cells(cv, f[k].x, f[k].y);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (ib + k < i1) {
+        const int a = classify_cells(pc[k], f[k].x, f[k].y, f[k].z, cycle, nspawn, mutual);
+        act[ib + k] = (unsigned char)a;
+        acc = add4(acc, pair_counts_n(f[k], a, mutual, pc[k].nci, pc[k].ncj));
+      }
     }
-    int4 tot;
-    block_exscan4(c, &tot);
-    acc = add4(acc, tot);
   }
-  if (threadIdx.x == 0) blk[blockIdx.x] = acc;
+  int4 tot;
+  block_exscan4<NT>(acc, &tot);  // only the block total is kept
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
 // Second pass of a level.  Every block first scans the kTravGrid block
@@ -334,28 +408,31 @@ __global__ void __launch_bounds__(kTravBlock)
 // scan launch), which gives its output offsets, the level's totals and the
 // capacity check (the same for every block); then it emits its chunk.  The
 // last block to finish moves the state to the next level.
-__global__ void __launch_bounds__(kTravBlock)
-    k_emit_dev(CellView cv, int4 *__restrict__ fa, int4 *__restrict__ fb, TravState *st, int mutual,
-               const int *__restrict__ act, const int4 *__restrict__ blk, long long cap_m2l, long long cap_p2p,
-               int cap_f, int *__restrict__ m2l_i, int *__restrict__ m2l_j, unsigned char *__restrict__ m2l_img,
-               int *__restrict__ p2p_i, int *__restrict__ p2p_j, unsigned char *__restrict__ p2p_img) {
-  constexpr int PER = kTravGrid / kTravBlock;
+template <int NT, int K>
+__global__ void __launch_bounds__(NT, NT / 128)  // two blocks per CU
+    k_emit_dev(CellView cv, int2 *__restrict__ fa, int2 *__restrict__ fb, TravState *st, int mutual,
+               const unsigned char *__restrict__ act, const int4 *__restrict__ blk, long long cap_m2l,
+               long long cap_p2p, int cap_f, int *__restrict__ m2l_i, int *__restrict__ m2l_j,
+               unsigned char *__restrict__ m2l_img, int *__restrict__ p2p_i, int *__restrict__ p2p_j,
+               unsigned char *__restrict__ p2p_img) {
+  constexpr int PER = (kTravGrid + NT - 1) / NT;
   __shared__ int s_last;
   __shared__ int4 s_run;
   const int nf = st->nf;
   const int fsel = st->fsel;
   const long long bm = st->nm, bp = st->np;
-  const int4 *fr = fsel ? fb : fa;
-  int4 *next = fsel ? fa : fb;
+  const int2 *fr = fsel ? fb : fa;
+  int2 *next = fsel ? fa : fb;
   int4 tot;
   {
     int4 v[PER];
     int4 sum = make_int4(0, 0, 0, 0);
     for (int k = 0; k < PER; k++) {
-      v[k] = blk[threadIdx.x * PER + k];
+      const int b = threadIdx.x * PER + k;
+      v[k] = b < kTravGrid ? blk[b] : make_int4(0, 0, 0, 0);
       sum = add4(sum, v[k]);
     }
-    int4 r = block_exscan4(sum, &tot);
+    int4 r = block_exscan4<NT>(sum, &tot);
     for (int k = 0; k < PER; k++) {
       if (threadIdx.x * PER + k == (int)blockIdx.x) s_run = r;
       r = add4(r, v[k]);
@@ -368,19 +445,33 @@ __global__ void __launch_bounds__(kTravBlock)
   chunk_of(nf, i0, i1);
   if (!emit) i1 = i0;
   int4 run = s_run;
-  for (int s = i0; s < i1; s += kTravBlock) {
-    const int i = s + threadIdx.x;
-    int4 f = make_int4(0, 0, 0, 0), c = make_int4(0, 0, 0, 0);
-    int a = -1;
-    if (i < i1) {
-      f = fr[i];
-      a = act[i];
-      c = pair_counts(cv, f, a, mutual);
+  for (int s = i0; s < i1; s += NT * K) {
+    const int ib = s + threadIdx.x * K;
+    int4 f[K], lk[K];
+    int a[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const bool in = ib + k < i1;
+      f[k] = in ? unpack_pair(fr[ib + k]) : make_int4(0, 0, 0, 0);
+      a[k] = in ? act[ib + k] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) lk[k] = a[k] >= 2 ? load_links(cv, f[k]) : make_int4(0, 0, 0, 0);
+    int4 c[K], sum = make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      c[k] = pair_counts_n(f[k], a[k], mutual, lk[k].y, lk[k].w);
+      sum = add4(sum, c[k]);
     }
     int4 t4;
-    const int4 ofs = add4(block_exscan4(c, &t4), run);
+    int4 ofs = add4(block_exscan4<NT>(sum, &t4), run);
     run = add4(run, t4);
-    if (i < i1) emit_pair(cv, f, a, mutual, ofs, bm, bp, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (a[k] >= 0)
+        emit_pair_n(f[k], a[k], lk[k], mutual, ofs, bm, bp, m2l_i, m2l_j, m2l_img, p2p_i, p2p_j, p2p_img, next);
+      ofs = add4(ofs, c[k]);
+    }
   }
   // The last block to finish (every block has read the state by then) moves
   // the state to the next level; the next kernel sees it.
@@ -402,20 +493,20 @@ __global__ void __launch_bounds__(kTravBlock)
   }
 }
 
-__global__ void k_init_frontier(int4 *__restrict__ fr, int images) {
+__global__ void k_init_frontier(int2 *__restrict__ fr, int images) {
   const int i = threadIdx.x;
   if (images == 0) {
-    if (i == 0) fr[0] = make_int4(0, 0, 13, 0);
+    if (i == 0) fr[0] = pack_pair(0, 0, 13);
   } else if (i < 27) {
-    fr[i] = make_int4(0, 0, i, 0);
+    fr[i] = pack_pair(0, 0, i);
   }
 }
 
 // A forest (batched systems): every root against itself, no images; the
 // traversal never pairs cells of two systems.
-__global__ void k_init_frontier_roots(int4 *__restrict__ fr, const int *__restrict__ roots, int nroots) {
+__global__ void k_init_frontier_roots(int2 *__restrict__ fr, const int *__restrict__ roots, int nroots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nroots) fr[i] = make_int4(roots[i], roots[i], 13, 0);
+  if (i < nroots) fr[i] = pack_pair(roots[i], roots[i], 13);
 }
 
 // CSR offsets from the target-sorted keys: off[c] = first index with key >= c.
@@ -539,20 +630,23 @@ static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutua
   const long long cap_p = (long long)std::min(std::min(h.raw_pi.cap, h.raw_pj.cap), h.raw_pimg.cap) - 1;
   const int cap_fr = (int)std::min(std::min(h.fa.cap, h.fb.cap), h.ia.cap) - 1;  // frontier and action slots
   int4 *blk = reinterpret_cast<int4 *>(h.ib.p);
+  unsigned char *act = reinterpret_cast<unsigned char *>(h.ia.p);
   hipLaunchKernelGGL(k_trav_init, dim3(1), dim3(1), 0, h.stream, st, nf0);
   hipLaunchKernelGGL(k_trav_small, dim3(1), dim3(kSmallBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st, h.cfg.cycle,
-                     h.cfg.nspawn, mutual, h.ia.p, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
+                     h.cfg.nspawn, mutual, act, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p,
                      h.raw_pimg.p, cap_m, cap_p, cap_fr);
   TravState hs{};
   int enq = 0;
-  int batch = h.trav_levels > 0 ? h.trav_levels + 1 : 8;  // levels after the small ones
+  // The levels after the small ones: as many as the last traversal had (its
+  // last level leaves an empty frontier), then more in batches if needed.
+  int batch = h.trav_levels > 0 ? h.trav_levels : 8;
   while (true) {
     for (int l = 0; l < batch; l++) {
-      hipLaunchKernelGGL(k_classify_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
-                         h.cfg.cycle, h.cfg.nspawn, mutual, h.ia.p, blk);
-      hipLaunchKernelGGL(k_emit_dev, dim3(kTravGrid), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, h.fb.p, st,
-                         mutual, h.ia.p, blk, cap_m, cap_p, cap_fr, h.raw_mi.p, h.raw_mj.p, h.raw_mimg.p, h.raw_pi.p,
-                         h.raw_pj.p, h.raw_pimg.p);
+      hipLaunchKernelGGL((k_classify_dev<kClassifyBlock, 1>), dim3(kTravGrid), dim3(kClassifyBlock), 0, h.stream, cv,
+                         h.fa.p, h.fb.p, st, h.cfg.cycle, h.cfg.nspawn, mutual, act, blk);
+      hipLaunchKernelGGL((k_emit_dev<kEmitBlock, kEmitItems>), dim3(kTravGrid), dim3(kEmitBlock), 0, h.stream, cv,
+                         h.fa.p, h.fb.p, st, mutual, act, blk, cap_m, cap_p, cap_fr, h.raw_mi.p, h.raw_mj.p,
+                         h.raw_mimg.p, h.raw_pi.p, h.raw_pj.p, h.raw_pimg.p);
     }
     enq += batch;
     enqueue_multipoles(h);  // the second stream's work, while these levels run
@@ -571,6 +665,8 @@ static bool traverse_on_device(Handle &h, const CellView &cv, int nf0, int mutua
 
 void run_lists(Handle &h) {
   const int T = 256;
+  if (h.ncells >= (1 << kImgShift))
+    throw Error(EXAFMM_HIP_EUNSUP, "traversal: more than 2^27 cells (frontier pairs hold 27-bit cell indices)");
   CellView cv{h.cX.p, h.cChild.p, h.cNChild.p, h.cNBody.p};
   // Raw (emission-order) lists, kept on the handle across evaluations.
   DBuf<int> &mi = h.raw_mi, &mj = h.raw_mj, &pi = h.raw_pi, &pj = h.raw_pj;
@@ -615,7 +711,7 @@ void run_lists(Handle &h) {
     h.ia.reserve(nf + 1);  // action
     h.ib.reserve(4 * ((size_t)nb + 1));  // block totals (int4)
     h.ic.reserve(4 * ((size_t)nb + 1));  // block offsets (int4)
-    int *act = h.ia.p;
+    unsigned char *act = reinterpret_cast<unsigned char *>(h.ia.p);
     int4 *blk = reinterpret_cast<int4 *>(h.ib.p), *blkofs = reinterpret_cast<int4 *>(h.ic.p);
     hipLaunchKernelGGL(k_classify, dim3(nb), dim3(kTravBlock), 0, h.stream, cv, h.fa.p, nf, h.cfg.cycle,
                        h.cfg.nspawn, mutual, act, blk);
