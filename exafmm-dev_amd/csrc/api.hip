@@ -174,6 +174,7 @@ int exafmm_hip_create(const exafmm_hip_config *cfg, exafmm_hip_handle **out) {
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_cnt, hipEventDisableTiming));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_lists, hipEventDisableTiming));
     EXA_HIP(hipEventCreateWithFlags(&H->h.ev_p2pcnt, hipEventDisableTiming));
+    EXA_HIP(hipEventCreateWithFlags(&H->h.ev_bounds, hipEventDisableTiming));
     EXA_HIP(hipHostMalloc((void **)&H->h.hpin, Handle::kPinWords * sizeof(int), hipHostMallocDefault));
   } catch (const Error &e) {
     delete H;
@@ -247,6 +248,7 @@ int exafmm_hip_get_bounds(exafmm_hip_handle *H, double *xmin3, double *xmax3) {
   return guarded(H, [&](Handle &h) {
     drop_graph(h);
     run_bounds(h);
+    sync_bounds(h);
     for (int d = 0; d < 3; d++) {
       if (xmin3) xmin3[d] = h.xmin[d];
       if (xmax3) xmax3[d] = h.xmax[d];
@@ -258,10 +260,12 @@ int exafmm_hip_build_tree(exafmm_hip_handle *H, const double *xmin3, const doubl
   return guarded(H, [&](Handle &h) {
     drop_graph(h);
     if (xmin3 && xmax3) {
+      sync_bounds(h);  // a copy still in flight would overwrite these
       for (int d = 0; d < 3; d++) {
         h.xmin[d] = xmin3[d];
         h.xmax[d] = xmax3[d];
       }
+      h.bounds_dev_stale = true;
       h.have_bounds = true;
     }
     do_build(h);

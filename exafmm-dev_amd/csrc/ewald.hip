@@ -284,6 +284,7 @@ void run_ewald(Handle &h, int ksize, double alpha, double sigma, double cutoff, 
     // The ancestor walk can be skipped when 3 cutoff exceeds every wrapped
     // centre distance: a wrapped coordinate difference is at most
     // max(cycle / 2, extent - cycle) for bodies spanning `extent`.
+    sync_bounds(h);
     double ext = 0;
     for (int d = 0; d < 3; d++) ext = std::max(ext, h.xmax[d] - h.xmin[d]);
     const double bound = std::sqrt(3.0) * std::max(0.5 * cycle, ext - cycle);

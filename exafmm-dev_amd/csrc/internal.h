@@ -91,8 +91,10 @@ struct Handle {
   hipEvent_t ev_cnt = nullptr;   // a device count copied into hpin
   hipEvent_t ev_lists = nullptr; // raw lists done (the P2P grouping on stream2 waits for it)
   hipEvent_t ev_p2pcnt = nullptr; // P2P target count copied into hpin[9] (stream2)
+  hipEvent_t ev_bounds = nullptr; // bounds copied into hpin[kPinBounds..] (main stream)
   int *hpin = nullptr;           // pinned host words for counts read back behind queued work
   static constexpr int kPinWords = 512;  // hpin: [0..64) counts, [64..322) lattice tile histogram, ...
+  static constexpr int kPinBounds = 400;  // hpin: [400, 412) the six bounds (doubles)
 
   // Bodies.
   int n = 0;
@@ -112,6 +114,12 @@ struct Handle {
   DBuf<double4> trg4;    // tree order results (pot, gx, gy, gz)
   DBuf<float4> trgf;     // scratch of the mixed-precision P2P
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
+  // Device copy: [0, 6) the bounds (xmin, xmax), [8, 12) the uniform box
+  // (x0, y0, z0, leaf cell size).  bounds_pending: xmin/xmax still wait for
+  // the copy behind ev_bounds (sync_bounds); bounds_dev_stale: the host
+  // values were set by the caller and the device copy is out of date.
+  DBuf<double> box;
+  bool bounds_pending = false, bounds_dev_stale = false;
   bool have_bounds = false, have_tree = false, have_up = false, have_trav = false;
 
   // Cells (reference Cell order).
@@ -271,6 +279,7 @@ struct CellSetView {
 
 // Phase implementations (one translation unit each).
 void run_bounds(Handle &h);
+void sync_bounds(Handle &h);
 void run_build_uniform(Handle &h);
 void run_build_forest(Handle &h);
 void run_build_adaptive(Handle &h);

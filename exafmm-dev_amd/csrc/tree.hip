@@ -71,24 +71,47 @@ __global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__
   if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
+// The bounds stay on the device for the builder; the host copy (xmin,
+// xmax) arrives behind ev_bounds and is read by sync_bounds when needed.
 void run_bounds(Handle &h) {
+  h.box.reserve(16);
+  h.bounds_dev_stale = false;
   if (h.n == 0) {
+    sync_bounds(h);
     for (int d = 0; d < 3; d++) h.xmin[d] = h.xmax[d] = 0;
+    h.bounds_dev_stale = true;
     h.have_bounds = true;
     return;
   }
   const int nb = std::min(kBoundsBlocks, div_up(h.n, 256));
-  h.red.reserve(6 * nb + 6);
+  h.red.reserve(6 * nb);
   hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, h.stream, h.in4.p, h.n, h.red.p);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, h.stream, h.red.p, nb, h.red.p + 6 * nb);
-  double r[6];
-  EXA_HIP(hipMemcpyAsync(r, h.red.p + 6 * nb, sizeof(r), hipMemcpyDeviceToHost, h.stream));
-  EXA_HIP(hipStreamSynchronize(h.stream));
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, h.stream, h.red.p, nb, h.box.p);
+  EXA_HIP(hipMemcpyAsync(h.hpin + Handle::kPinBounds, h.box.p, 6 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+  EXA_HIP(hipEventRecord(h.ev_bounds, h.stream));
+  h.bounds_pending = true;
+  h.have_bounds = true;
+}
+
+void sync_bounds(Handle &h) {
+  if (!h.bounds_pending) return;
+  EXA_HIP(hipEventSynchronize(h.ev_bounds));
+  const double *r = reinterpret_cast<const double *>(h.hpin + Handle::kPinBounds);
   for (int d = 0; d < 3; d++) {
     h.xmin[d] = r[d];
     h.xmax[d] = r[3 + d];
   }
-  h.have_bounds = true;
+  h.bounds_pending = false;
+}
+
+// The device bounds set from the host values (bounds given by the caller).
+static void upload_bounds(Handle &h) {
+  h.box.reserve(16);
+  if (!h.bounds_dev_stale) return;
+  const double v[6] = {h.xmin[0], h.xmin[1], h.xmin[2], h.xmax[0], h.xmax[1], h.xmax[2]};
+  EXA_HIP(hipMemcpyAsync(h.box.p, v, sizeof(v), hipMemcpyHostToDevice, h.stream));
+  EXA_HIP(hipStreamSynchronize(h.stream));  // v lives on this stack frame
+  h.bounds_dev_stale = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -98,12 +121,32 @@ void run_bounds(Handle &h) {
 // the division, truncation to int, bit interleave in int.
 // (The key has 3 * level <= 30 bits: it is kept as a 32-bit int for the
 // radix sort; k_widen_keys turns the sorted keys into the 64-bit form.)
-__global__ void k_morton32(const double4 *__restrict__ in, int n, double x0, double y0, double z0, float d, int level,
+// bounds2box (build_tree_omp.h:21-33) on the device: bb = the bounds
+// (xmin, xmax); box = (x0, y0, z0, d) with d the float leaf cell size of a
+// depth-L grid (:218-220), the same double operations as the reference.
+__global__ void k_box(const double *__restrict__ bb, int L, double *__restrict__ box) {
+  double Xc[3], R0 = 0;
+  for (int d = 0; d < 3; d++) Xc[d] = (bb[3 + d] + bb[d]) / 2;
+  for (int d = 0; d < 3; d++) {  // std::max(a, R0) = a < R0 ? R0 : a
+    const double a = Xc[d] - bb[d], b = bb[3 + d] - Xc[d];
+    R0 = a < R0 ? R0 : a;
+    R0 = b < R0 ? R0 : b;
+  }
+  R0 *= 1.00001;
+  const float d = 2 * R0 / (1 << L);
+  box[0] = bb[0];
+  box[1] = bb[1];
+  box[2] = bb[2];
+  box[3] = d;
+}
+
+__global__ void k_morton32(const double4 *__restrict__ in, int n, const double *__restrict__ box, int level,
                            int *__restrict__ key, int *__restrict__ idx) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const double4 v = in[b];
-  const double dd = (double)d;
+  const double x0 = box[0], y0 = box[1], z0 = box[2];
+  const double dd = (double)(float)box[3];
   int ix = (int)((v.x - x0) / dd);
   int iy = (int)((v.y - y0) / dd);
   int iz = (int)((v.z - z0) / dd);
@@ -187,7 +230,6 @@ struct LevelArgs {
   int count, count_up, count_dn;
   int off, off_up, off_dn;
   int shift;           // 3 * (L - t)
-  float d;             // cell size of this level
   int leaf;            // 1 at the finest level
   int level;
 };
@@ -196,7 +238,7 @@ struct LevelArgs {
 // arithmetic of reverseOrder: ascending rank r at level t lands at
 // off_t + count_t - 1 - r.
 __device__ inline void level_cell(const LevelArgs &a, const double4 *__restrict__ body,
-                                  const uint64_t *__restrict__ key, int n, int nc, double x0, double y0, double z0,
+                                  const uint64_t *__restrict__ key, int n, int nc, const double *__restrict__ box,
                                   double4 *__restrict__ cX, double *__restrict__ cRb, int *__restrict__ cParent,
                                   int *__restrict__ cChild, int *__restrict__ cNChild, int *__restrict__ cBody,
                                   int *__restrict__ cNBody, uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
@@ -206,7 +248,10 @@ __device__ inline void level_cell(const LevelArgs &a, const double4 *__restrict_
   const int end = (r + 1 < a.count) ? a.pos[r + 1] : n;
   const int c = a.off + a.count - 1 - r;
   const double4 v = body[start];
-  const double dd = (double)a.d;
+  const double x0 = box[0], y0 = box[1], z0 = box[2];
+  float dt = (float)box[3];
+  for (int l = 0; l < a.shift / 3; l++) dt *= 2;  // leafs2cells doubles d per level (:166)
+  const double dd = (double)dt;
   const int ix = (int)((v.x - x0) / dd);
   const int iy = (int)((v.y - y0) / dd);
   const int iz = (int)((v.z - z0) / dd);
@@ -243,12 +288,11 @@ struct LevelArgsAll {
 
 // k_level_cells of every level in one launch (level t = blockIdx.y).
 __global__ void k_level_cells_all(LevelArgsAll all, const double4 *__restrict__ body, const uint64_t *__restrict__ key,
-                                  int n, int nc, double x0, double y0, double z0, double4 *__restrict__ cX,
+                                  int n, int nc, const double *__restrict__ box, double4 *__restrict__ cX,
                                   double *__restrict__ cRb, int *__restrict__ cParent, int *__restrict__ cChild,
                                   int *__restrict__ cNChild, int *__restrict__ cBody, int *__restrict__ cNBody,
                                   uint64_t *__restrict__ cKey, int *__restrict__ cLevel) {
-  level_cell(all.a[blockIdx.y], body, key, n, nc, x0, y0, z0, cX, cRb, cParent, cChild, cNChild, cBody, cNBody, cKey,
-             cLevel);
+  level_cell(all.a[blockIdx.y], body, key, n, nc, box, cX, cRb, cParent, cChild, cNChild, cBody, cNBody, cKey, cLevel);
 }
 
 __global__ void k_iota(int *__restrict__ a, int n) {
@@ -262,15 +306,8 @@ void run_build_uniform(Handle &h) {
   // build_tree_omp.h:218 (integer n / ncrit inside the log).
   const int L = n >= ncrit ? 1 + int(std::log((double)(n / ncrit)) / M_LN2 / 3) : 0;
   if (L > 10) throw Error(EXAFMM_HIP_EUNSUP, "uniform tree deeper than 10 levels overflows the reference's int keys");
-  // bounds2box (build_tree_omp.h:21-33).
-  double Xc[3], R0 = 0;
-  for (int d = 0; d < 3; d++) Xc[d] = (h.xmax[d] + h.xmin[d]) / 2;
-  for (int d = 0; d < 3; d++) {
-    R0 = std::max(Xc[d] - h.xmin[d], R0);
-    R0 = std::max(h.xmax[d] - Xc[d], R0);
-  }
-  R0 *= 1.00001;
-  const float d = 2 * R0 / (1 << L);
+  upload_bounds(h);
+  hipLaunchKernelGGL(k_box, dim3(1), dim3(1), 0, h.stream, h.box.p, L, h.box.p + 8);
 
   h.k64a.reserve(n + 1);
   h.k64b.reserve(n + 1);
@@ -284,8 +321,7 @@ void run_build_uniform(Handle &h) {
   // keys takes several times as many passes), then widened.  k64b holds the
   // two 32-bit key buffers.
   int *k32 = reinterpret_cast<int *>(h.k64b.p), *k32alt = k32 + (n + 1);
-  hipLaunchKernelGGL(k_morton32, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.xmin[0], h.xmin[1], h.xmin[2],
-                     d, L, k32, h.ia.p);
+  hipLaunchKernelGGL(k_morton32, dim3(div_up(n, T)), dim3(T), 0, h.stream, h.in4.p, n, h.box.p + 8, L, k32, h.ia.p);
   bool alt = false;
   sort_pairs_int_int(h, k32, k32alt, h.ia.p, h.ib.p, n, 3 * L, alt);
   hipLaunchKernelGGL(k_widen_keys, dim3(div_up(n, T)), dim3(T), 0, h.stream, alt ? k32alt : k32, n, h.k64a.p);
@@ -313,6 +349,7 @@ void run_build_uniform(Handle &h) {
   hipLaunchKernelGGL(k_strided_gather, dim3(1), dim3(64), 0, h.stream, h.id.p + n, (int64_t)stride, L + 1, h.cnt.p);
   EXA_HIP(hipMemcpyAsync(tot.data(), h.cnt.p, (L + 1) * sizeof(int), hipMemcpyDeviceToHost, h.stream));
   EXA_HIP(hipStreamSynchronize(h.stream));
+  sync_bounds(h);  // (arrived before the totals)
   int nc = 0;
   std::vector<int> off(L + 2, 0);
   for (int t = 0; t <= L; t++) {
@@ -358,15 +395,12 @@ void run_build_uniform(Handle &h) {
     a.off_up = t > 0 ? off[t - 1] : 0;
     a.off_dn = t < L ? off[t + 1] : 0;
     a.shift = 3 * (L - t);
-    float dt = d;
-    for (int l = 0; l < L - t; l++) dt *= 2;  // leafs2cells doubles d per level (:166)
-    a.d = dt;
     a.leaf = (t == L);
     a.level = t;
     maxcount = std::max(maxcount, counts[t]);
   }
   hipLaunchKernelGGL(k_level_cells_all, dim3(div_up(maxcount, T), L + 1), dim3(T), 0, h.stream, all, h.body4.p, key, n,
-                     nc, h.xmin[0], h.xmin[1], h.xmin[2], h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p,
+                     nc, h.box.p + 8, h.cX.p, h.cRb.p, h.cParent.p, h.cChild.p, h.cNChild.p,
                      h.cBody.p, h.cNBody.p, h.cKey.p, h.cLevel.p);
   // Level order: the layout is already level-contiguous.
   h.lvl_off.assign(off.begin(), off.begin() + L + 2);
@@ -640,7 +674,6 @@ void run_build_forest(Handle &h) {
     a.off_up = t > 0 ? loff[t - 1] : 0;
     a.off_dn = t < L ? loff[t + 1] : 0;
     a.shift = 0;
-    a.d = 0;
     a.leaf = 0;
     a.level = t;
     hipLaunchKernelGGL(k_level_cells_forest, dim3(div_up(counts[t], T)), dim3(T), 0, h.stream, a, h.body4.p, key, n,

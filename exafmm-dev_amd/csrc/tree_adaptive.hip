@@ -304,6 +304,7 @@ __global__ void k_fill_level(int *__restrict__ level, int first, const int *__re
 void run_build_adaptive(Handle &h) {
   const int n = h.n, ncrit = h.cfg.ncrit;
   const int T = 256;
+  sync_bounds(h);
   // bounds2box (build_tree_tbb.h:298-310).
   double Xc[3], R0 = 0;
   for (int d = 0; d < 3; d++) Xc[d] = (h.xmax[d] + h.xmin[d]) / 2;

@@ -64,12 +64,14 @@ Handle::~Handle() {
   fb.release();
   cnt.release();
   red.release();
+  box.release();
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (ev_mult) (void)hipEventDestroy(ev_mult);
   if (ev_cnt) (void)hipEventDestroy(ev_cnt);
   if (ev_lists) (void)hipEventDestroy(ev_lists);
   if (ev_p2pcnt) (void)hipEventDestroy(ev_p2pcnt);
+  if (ev_bounds) (void)hipEventDestroy(ev_bounds);
   if (hpin) (void)hipHostFree(hpin);
   if (stream) (void)hipStreamDestroy(stream);
   if (stream2) (void)hipStreamDestroy(stream2);
