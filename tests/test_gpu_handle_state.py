@@ -7,6 +7,9 @@
 * set_config invalidates the passes that depend on a changed parameter:
   after a change of theta the traversal refuses to run until the upward
   pass has been rerun, and the rerun equals a fresh handle bit for bit.
+* Bounds stay on the device for the builder: bounds a caller passes to
+  build_tree replace those an evaluation computed, and get_bounds returns
+  the computed ones; every tree equals a fresh handle's bit for bit.
 """
 import numpy as np
 import pytest
@@ -77,3 +80,36 @@ def test_set_config_invalidates_dependent_passes():
     assert np.all(np.isfinite(f.results()))
     f.close()
     g.close()
+
+
+def test_caller_bounds_replace_device_bounds():
+    """Bounds given to build_tree replace the ones an evaluation computed on
+    the device (whose host copy may still be in flight), and get_bounds
+    brings the computed ones back: each tree equals a fresh handle's."""
+    x = unit_cube(16384, 6)
+    print("[gpu] handle state: caller bounds after device bounds", flush=True)
+    f = X.FMM(P=8, ncrit=64)
+    f.set_bodies(x)
+    f.evaluate()
+    f.build_tree((np.full(3, -1.5), np.full(3, 1.25)))
+    cw = f.cells()
+    g = X.FMM(P=8, ncrit=64)
+    g.set_bodies(x)
+    g.build_tree((np.full(3, -1.5), np.full(3, 1.25)))
+    cg = g.cells()
+    for k in ("iparent", "ichild", "nchild", "ibody", "nbody", "icell", "X", "R"):
+        assert np.array_equal(cw[k], cg[k]), k
+    lo, hi = f.get_bounds()
+    assert np.array_equal(lo, x[:, :3].min(0)) and np.array_equal(hi, x[:, :3].max(0))
+    f.build_tree()
+    cd = f.cells()
+    h = X.FMM(P=8, ncrit=64)
+    h.set_bodies(x)
+    h.get_bounds()
+    h.build_tree()
+    ch = h.cells()
+    for k in ("iparent", "ichild", "nchild", "ibody", "nbody", "icell", "X", "R"):
+        assert np.array_equal(cd[k], ch[k]), k
+    assert not np.array_equal(cd["X"][:1], cw["X"][:1])  # the caller's box was used
+    for o in (f, g, h):
+        o.close()
