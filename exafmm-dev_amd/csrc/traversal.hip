@@ -11,6 +11,7 @@
 // recursion (mutual or not); only the order differs, and the lists are
 // regrouped by target cell (CSR) afterwards.
 #include <algorithm>
+#include <climits>
 
 #include "internal.h"
 
@@ -583,6 +584,7 @@ static void group_by_target(Handle &h, GroupCtx g, int64_t np, DBuf<int> &tgt, D
                             DBuf<int> &tlist, int *ntl_host, const int **ntl_dev = nullptr) {
   const int nc = h.ncells;
   const int T = 256;
+  if (np > INT_MAX) throw Error(EXAFMM_HIP_EUNSUP, "list grouping: more than 2^31 - 1 entries");
   off.resize(nc + 1);
   src_out.resize(np);
   img_out.resize(np);
