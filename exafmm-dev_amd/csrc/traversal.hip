@@ -31,7 +31,9 @@ __device__ inline void image_offset(int img, double cycle, double &px, double &p
 // A frontier pair packed in 8 bytes: (Ci, Cj | image << 27); the host keeps
 // cell indices below 2^27.
 constexpr int kImgShift = 27;
-__device__ inline int2 pack_pair(int ci, int cj, int img) { return make_int2(ci, cj | (img << kImgShift)); }
+__device__ inline int2 pack_pair(int ci, int cj, int img) {
+  return make_int2(ci, (int)((unsigned)cj | ((unsigned)img << kImgShift)));
+}
 __device__ inline int4 unpack_pair(int2 p) {
   return make_int4(p.x, p.y & ((1 << kImgShift) - 1), (int)((unsigned)p.y >> kImgShift), 0);
 }
