@@ -6,8 +6,9 @@ device-driven passes, and fall back to the per-level passes when an output
 would not fit.  Bar: the lists (in their emitted, target-grouped order) and
 the results of a later evaluation are identical, bit for bit, to those of a
 fresh handle -- for open and periodic boundaries, mutual traversal, the
-adaptive builder, batched forests, and a tree that outgrows the recorded
-capacities.
+adaptive builder, batched forests, a tree that outgrows the recorded
+capacities, and a deeper tree that fits them but needs more levels than the
+last traversal had.
 """
 import numpy as np
 import pytest
@@ -100,5 +101,28 @@ def test_batched_forest_repeats():
         b = f.evaluate_batch(systems)
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
+    finally:
+        f.close()
+
+
+def test_deeper_tree_within_the_recorded_capacity():
+    # A large shallow tree records large capacities and its level count; a
+    # smaller, deeper one then fits them but needs more levels than were
+    # enqueued, so the device-driven passes continue after the read-back.
+    wide, deep = cube(60000, 7), plummer(20000, 8)
+    ref = X.FMM(P=6, ncrit=32, builder="adaptive")
+    try:
+        want = snapshot(ref, deep)
+    finally:
+        ref.close()
+    f = X.FMM(P=6, ncrit=32, builder="adaptive")
+    try:
+        snapshot(f, wide)
+        snapshot(f, wide)
+        shallow_levels = f.stats()["num_levels"]
+        got = snapshot(f, deep)
+        assert f.stats()["num_levels"] > shallow_levels
+        same(got, want)
+        same(snapshot(f, deep), want)
     finally:
         f.close()
